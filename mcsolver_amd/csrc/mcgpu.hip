@@ -1,0 +1,1430 @@
+// libmcgpu — MI355X (gfx950) Monte Carlo sweep engine for Ising / XY /
+// Heisenberg spins on arbitrary bond graphs.  See include/mcgpu.h for the ABI.
+//
+// Kernels
+//   k_lds<NC,ST>      persistent sweep: each workgroup owns R replicas of one
+//                     lattice, stages their spins in LDS once, runs many
+//                     graph-coloured Metropolis sweeps with in-kernel fused
+//                     measurement, writes spins back once.  HBM traffic per
+//                     sweep ~0; bound by VALU/LDS.
+//   k_pass<NC,ST>     streaming colour pass over all replicas (spins in HBM /
+//                     L2), for lattices too large for LDS; also the "plain"
+//                     one-launch-per-replica-per-colour baseline.
+//   k_measure<NC,ST>  one workgroup per replica: FP64 full-Hamiltonian
+//                     recomputation and/or one measured frame.
+//   k_wolff<NC,ST>    one workgroup per replica: level-synchronous frontier
+//                     BFS Wolff clusters (embedded-spin for O(n)).
+//   k_pt              parallel-tempering swaps of temperature rows.
+//
+// Every single-site update is the same deterministic function of
+// (spins, site, sweep index, replica key), with a counter-based Philox
+// stream, so all kernels produce the same trajectory bit for bit.
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mcgpu.h"
+#include "mc_math.h"
+
+using namespace mcg;
+
+// ---------------------------------------------------------------------------
+// error reporting
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return -1;
+}
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) return fail("%s failed: %s", #x, hipGetErrorString(e_));       \
+  } while (0)
+
+extern "C" const char* mcg_last_error(void) { return g_err.c_str(); }
+extern "C" int mcg_version(void) { return 1; }
+
+// ---------------------------------------------------------------------------
+// spin storage
+// ---------------------------------------------------------------------------
+template <typename ST>
+struct SIO;
+template <>
+struct SIO<float> {
+  static __device__ __forceinline__ float ld(const float* p) { return *p; }
+  static __device__ __forceinline__ void st(float* p, float v) { *p = v; }
+};
+template <>
+struct SIO<__half> {
+  static __device__ __forceinline__ float ld(const __half* p) { return __half2float(*p); }
+  static __device__ __forceinline__ void st(__half* p, float v) { *p = __float2half_rn(v); }
+};
+template <>
+struct SIO<int8_t> {
+  static __device__ __forceinline__ float ld(const int8_t* p) { return (float)*p; }
+  static __device__ __forceinline__ void st(int8_t* p, float v) { *p = (int8_t)v; }
+};
+
+template <typename ST>
+struct Spins {
+  ST* p;
+  int n;
+  __device__ __forceinline__ float get(int i, int c) const { return SIO<ST>::ld(p + (size_t)c * n + i); }
+  __device__ __forceinline__ void set(int i, int c, float v) const { SIO<ST>::st(p + (size_t)c * n + i, v); }
+};
+
+static constexpr double REF_PI = 3.1415926535;  // heisenbergLib.c:6
+
+// ---------------------------------------------------------------------------
+// the single-site Metropolis update
+// ---------------------------------------------------------------------------
+// Energy: E = sum_i [ 1/2 sum_l s_i^T J_il s_j(i,l) + sum_a D_a s_ia^2 - h s_i,axis ]
+// (axis z for Heisenberg, x for XY; Ising s = sigma*S).  Proposal: reflection
+// s' = s - 2 (s.r) r with r uniform on the sphere / circle
+// (heisenbergLib.c:441-473, xyLib.c:382-409); Ising: flip (isingLib.c:238-254).
+// Accept if beta*dE <= 0 or u < exp(-beta*dE).  Accepted O(n) spins are
+// renormalised to |S|.
+template <int NC, typename ST>
+__device__ __forceinline__ void metro_site(const Spins<ST>& sp, int i, const uint32_t* __restrict__ link, int zmax,
+                                           const float* coup, const uint8_t* __restrict__ cls, const float* clsdat,
+                                           float beta, float h, uint64_t sweep, uint32_t key0, uint32_t key1,
+                                           double& dEacc, double* dMacc) {
+  const int ci = cls[i];
+  const float S = clsdat[ci * 4 + 0];
+  const uint32_t* lk = link + (size_t)i * zmax;
+  const u32x4 x = philox4x32_10(u32x4{(uint32_t)i, (uint32_t)sweep, (uint32_t)(sweep >> 32), TAG_METRO}, key0, key1);
+  if constexpr (NC == 1) {
+    float f = 0.0f;
+    for (int l = 0; l < zmax; ++l) {
+      const uint32_t w = lk[l];
+      f = fmaf(coup[(w >> 24) * 9], sp.get((int)(w & 0xFFFFFFu), 0), f);
+    }
+    const float sg = sp.get(i, 0);
+    const float s = sg * S;
+    const float t = f - h;
+    const float dE = (-2.0f * s) * t;
+    const float db = dE * beta;
+    if (db <= 0.0f || u24(x.z) < exp_neg(-db)) {
+      sp.set(i, 0, -sg);
+      dEacc += (double)dE;
+      dMacc[0] += (double)(-2.0f * s);
+    }
+  } else {
+    const float Dx = clsdat[ci * 4 + 1], Dy = clsdat[ci * 4 + 2], Dz = clsdat[ci * 4 + 3];
+    const float s0 = sp.get(i, 0), s1 = sp.get(i, 1);
+    const float s2 = NC == 3 ? sp.get(i, 2) : 0.0f;
+    float H0 = 0.0f, H1 = 0.0f, H2 = 0.0f;
+    for (int l = 0; l < zmax; ++l) {
+      const uint32_t w = lk[l];
+      const int j = (int)(w & 0xFFFFFFu);
+      const float* J = coup + (w >> 24) * 9;
+      const float t0 = sp.get(j, 0), t1 = sp.get(j, 1);
+      if constexpr (NC == 3) {
+        const float t2 = sp.get(j, 2);
+        H0 = fmaf(J[0], t0, H0); H0 = fmaf(J[1], t1, H0); H0 = fmaf(J[2], t2, H0);
+        H1 = fmaf(J[3], t0, H1); H1 = fmaf(J[4], t1, H1); H1 = fmaf(J[5], t2, H1);
+        H2 = fmaf(J[6], t0, H2); H2 = fmaf(J[7], t1, H2); H2 = fmaf(J[8], t2, H2);
+      } else {
+        H0 = fmaf(J[0], t0, H0); H0 = fmaf(J[1], t1, H0);
+        H1 = fmaf(J[3], t0, H1); H1 = fmaf(J[4], t1, H1);
+      }
+    }
+    float rx, ry, rz = 0.0f;
+    if constexpr (NC == 3) {
+      rz = fmaf(u24(x.x), 2.0f, -1.0f);
+      const float rho = sqrtf(fmaxf(fmaf(-rz, rz, 1.0f), 0.0f));
+      float c, sn;
+      sincos_turn(x.y, &c, &sn);
+      rx = rho * c;
+      ry = rho * sn;
+    } else {
+      sincos_turn(x.y, &rx, &ry);
+    }
+    float p = s0 * rx;
+    p = fmaf(s1, ry, p);
+    if constexpr (NC == 3) p = fmaf(s2, rz, p);
+    const float a = -2.0f * p;
+    const float d0 = a * rx, d1 = a * ry, d2 = a * rz;
+    float dE = d0 * H0;
+    dE = fmaf(d1, H1, dE);
+    if constexpr (NC == 3) dE = fmaf(d2, H2, dE);
+    float n0 = s0 + d0, n1 = s1 + d1, n2 = s2 + d2;
+    {
+      const float a0 = n0 * n0, b0 = s0 * s0, q0 = a0 - b0;
+      const float a1 = n1 * n1, b1 = s1 * s1, q1 = a1 - b1;
+      dE = fmaf(Dx, q0, dE);
+      dE = fmaf(Dy, q1, dE);
+      if constexpr (NC == 3) {
+        const float a2 = n2 * n2, b2 = s2 * s2, q2 = a2 - b2;
+        dE = fmaf(Dz, q2, dE);
+      }
+    }
+    dE = fmaf(-h, NC == 3 ? d2 : d0, dE);
+    const float db = dE * beta;
+    if (db <= 0.0f || u24(x.z) < exp_neg(-db)) {
+      float nn = n0 * n0;
+      nn = fmaf(n1, n1, nn);
+      if constexpr (NC == 3) nn = fmaf(n2, n2, nn);
+      const float f = fabsf(S) / sqrtf(nn);
+      n0 = n0 * f;
+      n1 = n1 * f;
+      n2 = n2 * f;
+      sp.set(i, 0, n0);
+      sp.set(i, 1, n1);
+      if constexpr (NC == 3) sp.set(i, 2, n2);
+      dEacc += (double)dE;
+      dMacc[0] += (double)(sp.get(i, 0) - s0);
+      dMacc[1] += (double)(sp.get(i, 1) - s1);
+      if constexpr (NC == 3) dMacc[2] += (double)(sp.get(i, 2) - s2);
+    }
+  }
+}
+
+// spin vector of site i (Ising: sigma*S)
+template <int NC, typename ST>
+__device__ __forceinline__ void spin_vec(const Spins<ST>& sp, int i, const uint8_t* __restrict__ cls, const float* clsdat,
+                                         float* v) {
+  if constexpr (NC == 1) {
+    v[0] = sp.get(i, 0) * clsdat[cls[i] * 4];
+    v[1] = v[2] = 0.0f;
+  } else {
+    v[0] = sp.get(i, 0);
+    v[1] = sp.get(i, 1);
+    v[2] = NC == 3 ? sp.get(i, 2) : 0.0f;
+  }
+}
+
+// per-site energy e_i (FP32): 1/2 s_i.H_i + onsite - h s_axis
+template <int NC, typename ST>
+__device__ __forceinline__ float site_energy(const Spins<ST>& sp, int i, const uint32_t* __restrict__ link, int zmax,
+                                             const float* coup, const uint8_t* __restrict__ cls, const float* clsdat,
+                                             float h) {
+  const int ci = cls[i];
+  const uint32_t* lk = link + (size_t)i * zmax;
+  if constexpr (NC == 1) {
+    float f = 0.0f;
+    for (int l = 0; l < zmax; ++l) {
+      const uint32_t w = lk[l];
+      f = fmaf(coup[(w >> 24) * 9], sp.get((int)(w & 0xFFFFFFu), 0), f);
+    }
+    const float s = sp.get(i, 0) * clsdat[ci * 4];
+    return fmaf(0.5f * s, f, -h * s);
+  } else {
+    const float s0 = sp.get(i, 0), s1 = sp.get(i, 1), s2 = NC == 3 ? sp.get(i, 2) : 0.0f;
+    float H0 = 0.0f, H1 = 0.0f, H2 = 0.0f;
+    for (int l = 0; l < zmax; ++l) {
+      const uint32_t w = lk[l];
+      const int j = (int)(w & 0xFFFFFFu);
+      const float* J = coup + (w >> 24) * 9;
+      const float t0 = sp.get(j, 0), t1 = sp.get(j, 1), t2 = NC == 3 ? sp.get(j, 2) : 0.0f;
+      H0 = fmaf(J[0], t0, H0); H0 = fmaf(J[1], t1, H0); H0 = fmaf(J[2], t2, H0);
+      H1 = fmaf(J[3], t0, H1); H1 = fmaf(J[4], t1, H1); H1 = fmaf(J[5], t2, H1);
+      H2 = fmaf(J[6], t0, H2); H2 = fmaf(J[7], t1, H2); H2 = fmaf(J[8], t2, H2);
+    }
+    float e = s0 * H0;
+    e = fmaf(s1, H1, e);
+    e = fmaf(s2, H2, e);
+    e = 0.5f * e;
+    e = fmaf(clsdat[ci * 4 + 1] * s0, s0, e);
+    e = fmaf(clsdat[ci * 4 + 2] * s1, s1, e);
+    if constexpr (NC == 3) e = fmaf(clsdat[ci * 4 + 3] * s2, s2, e);
+    e = fmaf(-h, NC == 3 ? s2 : s0, e);
+    return e;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// group reductions (a group = TPR consecutive threads, TPR a multiple of 64)
+// ---------------------------------------------------------------------------
+static constexpr int RK = 8;  // values per reduction call
+
+__device__ __forceinline__ void group_sum8(double* v, double* s_red, int g, int TPR) {
+#pragma unroll
+  for (int k = 0; k < RK; ++k) {
+    double x = v[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    v[k] = x;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int k = 0; k < RK; ++k) s_red[wave * RK + k] = v[k];
+  __syncthreads();
+  if ((int)threadIdx.x == g * TPR) {
+    const int w0 = g * TPR / 64, nw = TPR / 64;
+    for (int k = 0; k < RK; ++k) {
+      double t = 0.0;
+      for (int q = 0; q < nw; ++q) t += s_red[(w0 + q) * RK + k];
+      v[k] = t;
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// measurement of one frame (all flagged observables) for one replica
+// ---------------------------------------------------------------------------
+struct FrameCtx {
+  const mcg_graph* G;
+  const float* coup;
+  const float* clsdat;
+  double* s_red;   // reduction scratch (nwaves*RK)
+  double* s_frm;   // per-group frame values (>= 64 doubles per group)
+  int g, lt, TPR;
+};
+
+// majority spin of block-spin site (heisenbergLib.c:255-267, isingLib.c:133-150)
+template <int NC, typename ST>
+__device__ __forceinline__ void majority(const Spins<ST>& sp, const mcg_graph& G, const float* clsdat, int site,
+                                         uint64_t frame, uint32_t key0, uint32_t key1, float* v) {
+  const int ri = G.rrank[site];
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  for (int k = 0; k < G.nclu; ++k) {
+    float t[3];
+    spin_vec<NC, ST>(sp, G.rclu[ri * G.nclu + k], G.cls, clsdat, t);
+    a0 += t[0];
+    a1 += t[1];
+    a2 += t[2];
+  }
+  const float S = clsdat[G.cls[site] * 4];
+  if constexpr (NC == 1) {
+    float sg;
+    if (a0 > 0.0f)
+      sg = 1.0f;
+    else if (a0 < 0.0f)
+      sg = -1.0f;
+    else {
+      const u32x4 x = philox4x32_10(u32x4{(uint32_t)site, (uint32_t)frame, (uint32_t)(frame >> 32), TAG_MAJORITY},
+                                    key0, key1);
+      sg = u24(x.x) > 0.5f ? 1.0f : -1.0f;
+    }
+    v[0] = sg * fabsf(S);
+    v[1] = v[2] = 0.0f;
+  } else {
+    const float l = sqrtf(a0 * a0 + a1 * a1 + a2 * a2);
+    if (l >= 1e-5f) {
+      a0 /= l;
+      a1 /= l;
+      a2 /= l;
+    }
+    v[0] = a0 * S;
+    v[1] = a1 * S;
+    v[2] = a2 * S;
+  }
+}
+
+// solid angle of a spin triangle, heisenbergLib.c:114-127 (atan, +-pi branch)
+__device__ __forceinline__ float signed_area(const float* a, const float* b, const float* c, float la, float lb,
+                                             float lc) {
+  const float ab = (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) / la / lb;
+  const float bc = (b[0] * c[0] + b[1] * c[1] + b[2] * c[2]) / lb / lc;
+  const float ca = (c[0] * a[0] + c[1] * a[1] + c[2] * a[2]) / lc / la;
+  const float x0 = b[1] * c[2] - b[2] * c[1], x1 = b[2] * c[0] - b[0] * c[2], x2 = b[0] * c[1] - b[1] * c[0];
+  const float re = 1.0f + ab + bc + ca;
+  const float im = (a[0] * x0 + a[1] * x1 + a[2] * x2) / la / lb / lc;
+  if (fabsf(re) < 1e-6f) return im > 0.0f ? (float)REF_PI : -(float)REF_PI;
+  return 2.0f * atanf(im / re);
+}
+
+// Measures one frame.  Tracked E and M (FP64) of the replica are passed in
+// st[]; everything else is computed from the spins.  Only the group leader
+// (lt == 0) writes the accumulators.
+template <int NC, typename ST>
+__device__ void measure_frame(const Spins<ST>& sp, const FrameCtx& C, const mcg_batch& B, const double* st, int row,
+                              int64_t frame, uint32_t key0, uint32_t key1, float h) {
+  const mcg_graph& G = *C.G;
+  const int lt = C.lt, TPR = C.TPR;
+  double* F = C.s_frm;  // leader-only use
+  double v[RK];
+  const int flags = B.meas;
+  // --- correlation pairs: S_i, S_j, sum s_i.s_j, last-site spin
+  if ((flags & MCG_MEAS_PAIRS) && G.nlat > 0) {
+    for (int k = 0; k < RK; ++k) v[k] = 0.0;
+    for (int p = lt; p < G.nlat; p += TPR) {
+      float a[3], b[3];
+      spin_vec<NC, ST>(sp, G.pairs[2 * p], G.cls, C.clsdat, a);
+      spin_vec<NC, ST>(sp, G.pairs[2 * p + 1], G.cls, C.clsdat, b);
+      v[0] += a[0]; v[1] += a[1]; v[2] += a[2];
+      v[3] += b[0]; v[4] += b[1]; v[5] += b[2];
+      v[6] += (double)(a[0] * b[0] + a[1] * b[1] + a[2] * b[2]);
+    }
+    if (lt == 0) {
+      float t[3];
+      spin_vec<NC, ST>(sp, G.n - 1, G.cls, C.clsdat, t);
+      v[7] = t[0];
+    }
+    group_sum8(v, C.s_red, C.g, TPR);
+    if (lt == 0)
+      for (int k = 0; k < 8; ++k) F[k] = v[k];
+  }
+  // --- topological charge
+  if ((flags & MCG_MEAS_Q) && NC == 3 && G.ncirc > 0) {
+    for (int k = 0; k < RK; ++k) v[k] = 0.0;
+    for (int p = lt; p < G.ncirc; p += TPR) {
+      const int ia = G.circ[3 * p], ib = G.circ[3 * p + 1], ic = G.circ[3 * p + 2];
+      float a[3], b[3], c[3];
+      spin_vec<NC, ST>(sp, ia, G.cls, C.clsdat, a);
+      spin_vec<NC, ST>(sp, ib, G.cls, C.clsdat, b);
+      spin_vec<NC, ST>(sp, ic, G.cls, C.clsdat, c);
+      v[0] += (double)signed_area(a, b, c, fabsf(C.clsdat[G.cls[ia] * 4]), fabsf(C.clsdat[G.cls[ib] * 4]),
+                                  fabsf(C.clsdat[G.cls[ic] * 4]));
+    }
+    group_sum8(v, C.s_red, C.g, TPR);
+    if (lt == 0) F[8] = v[0];
+  }
+  // --- block-spin observables
+  if ((flags & MCG_MEAS_RNORM) && G.nr > 0) {
+    for (int k = 0; k < RK; ++k) v[k] = 0.0;
+    double w[RK];
+    for (int k = 0; k < RK; ++k) w[k] = 0.0;
+    for (int p = lt; p < G.nlat; p += TPR) {
+      const int ia = G.pairs[2 * p], ib = G.pairs[2 * p + 1];
+      const bool ca = G.rrank[ia] >= 0, cb = G.rrank[ib] >= 0;
+      float ma[3] = {0, 0, 0}, mb[3] = {0, 0, 0};
+      if (ca) {
+        majority<NC, ST>(sp, G, C.clsdat, ia, frame, key0, key1, ma);
+        v[0] += ma[0]; v[1] += ma[1]; v[2] += ma[2];
+        w[0] += 1.0;
+      }
+      if (cb) {
+        majority<NC, ST>(sp, G, C.clsdat, ib, frame, key0, key1, mb);
+        v[3] += mb[0]; v[4] += mb[1]; v[5] += mb[2];
+        w[1] += 1.0;
+      }
+      if (ca && cb) {
+        v[6] += (double)(ma[0] * mb[0] + ma[1] * mb[1] + ma[2] * mb[2]);
+        w[2] += 1.0;
+      }
+    }
+    for (int ri = lt; ri < G.nr; ri += TPR) {
+      const int j = G.rorb[ri];
+      float m[3];
+      majority<NC, ST>(sp, G, C.clsdat, j, frame, key0, key1, m);
+      float cc = 0.0f;
+      for (int l = 0; l < G.zmax; ++l) {
+        const uint32_t wd = G.rlink[(size_t)ri * G.zmax + l];
+        const int t = (int)(wd & 0xFFFFFFu);
+        const float* J = C.coup + (wd >> 24) * 9;
+        float mt[3];
+        majority<NC, ST>(sp, G, C.clsdat, t, frame, key0, key1, mt);
+        if constexpr (NC == 1) {
+          cc += J[1] * m[0] * mt[0];
+        } else {
+          float H0 = J[0] * mt[0] + J[1] * mt[1] + J[2] * mt[2];
+          float H1 = J[3] * mt[0] + J[4] * mt[1] + J[5] * mt[2];
+          float H2 = J[6] * mt[0] + J[7] * mt[1] + J[8] * mt[2];
+          cc += m[0] * H0 + m[1] * H1 + m[2] * H2;
+        }
+      }
+      w[3] += (double)cc;
+      // on-site part with the site's actual spin (heisenbergLib.c:798-800,
+      // isingLib.c:390)
+      float s[3];
+      spin_vec<NC, ST>(sp, j, G.cls, C.clsdat, s);
+      const int ci = G.cls[j];
+      float eo;
+      if constexpr (NC == 1)
+        eo = -h * s[0];
+      else
+        eo = C.clsdat[ci * 4 + 1] * s[0] * s[0] + C.clsdat[ci * 4 + 2] * s[1] * s[1] +
+             C.clsdat[ci * 4 + 3] * s[2] * s[2] - h * (NC == 3 ? s[2] : s[0]);
+      w[4] += (double)eo;
+    }
+    group_sum8(v, C.s_red, C.g, TPR);
+    if (lt == 0)
+      for (int k = 0; k < 7; ++k) F[16 + k] = v[k];
+    group_sum8(w, C.s_red, C.g, TPR);
+    if (lt == 0)
+      for (int k = 0; k < 5; ++k) F[24 + k] = w[k];
+  }
+  // --- orbital groups (heisenbergLib.c:805-830): 2 groups per reduction
+  const int ng = (flags & MCG_MEAS_GROUPS) ? G.ngroup : 0;
+  for (int g0 = 0; g0 < ng; g0 += 2) {
+    for (int k = 0; k < RK; ++k) v[k] = 0.0;
+    for (int q = 0; q < 2 && g0 + q < ng; ++q) {
+      for (int k = lt; k < G.gmax; k += TPR) {
+        const int o = G.groups[(g0 + q) * G.gmax + k];
+        if (o < 0) continue;
+        float s[3];
+        spin_vec<NC, ST>(sp, o, G.cls, C.clsdat, s);
+        v[3 * q] += s[0]; v[3 * q + 1] += s[1]; v[3 * q + 2] += s[2];
+      }
+    }
+    group_sum8(v, C.s_red, C.g, TPR);
+    if (lt == 0)
+      for (int k = 0; k < 6; ++k) F[32 + 3 * g0 + k] = v[k];
+  }
+
+  if (lt != 0) return;
+  // --- leader: per-frame scalars -> FP64 accumulators (isingLib.c:388-431,
+  //     heisenbergLib.c:677-830 / xyLib.c:605-757)
+  double* A = B.acc + (size_t)row * MCG_NACC;
+  const double n = G.n;
+  const double nLat = G.nlat > 0 ? G.nlat : G.n;
+  const bool pairs = (flags & MCG_MEAS_PAIRS) && G.nlat > 0;
+  double E = st[MCG_S_E];
+  if (flags & MCG_ISING_RELE) E -= st[MCG_S_EREF];
+  const double e = E / n;
+  const double M0 = st[MCG_S_M], M1 = st[MCG_S_M + 1], M2 = st[MCG_S_M + 2];
+  double Mf;
+  const int ax = NC == 3 ? 2 : 0;
+  if constexpr (NC == 1) {
+    Mf = pairs ? F[0] / nLat : M0 / nLat;
+    if (pairs) {
+      A[MCG_A_SI] += fabs(F[0]) / nLat;
+      A[MCG_A_SJ] += fabs(F[3]) / nLat;
+      A[MCG_A_SIJ] += F[6] / nLat;
+      A[MCG_A_LAST] += F[7];
+    }
+    A[MCG_A_TOT] += M0;
+  } else {
+    const double Mv[3] = {M0, M1, NC == 3 ? M2 : 0.0};
+    double dir[3] = {Mv[0], Mv[1], Mv[2]};
+    const double ml = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    if (ml >= 1e-5) {
+      dir[0] /= ml; dir[1] /= ml; dir[2] /= ml;
+    }
+    if (NC == 3 || !pairs)
+      Mf = ml / nLat;
+    else
+      Mf = sqrt(F[0] * F[0] + F[1] * F[1]) / nLat;
+    if (pairs) {
+      for (int k = 0; k < NC; ++k) {
+        A[MCG_A_SI + k] += fabs(F[k] / nLat);
+        A[MCG_A_SJ + k] += fabs(F[3 + k] / nLat);
+      }
+      A[MCG_A_SIJ] += F[6] / nLat;
+      A[MCG_A_ZI] += (dir[0] * F[0] + dir[1] * F[1] + dir[2] * F[2]) / nLat;
+      A[MCG_A_ZJ] += (dir[0] * F[3] + dir[1] * F[4] + dir[2] * F[5]) / nLat;
+      A[MCG_A_HI] += F[ax] / nLat;
+      A[MCG_A_HJ] += F[3 + ax] / nLat;
+    }
+    A[MCG_A_ZT] += (dir[0] * Mv[0] + dir[1] * Mv[1] + dir[2] * Mv[2]) / nLat;
+    A[MCG_A_HT] += Mv[ax] / nLat;
+    if ((flags & MCG_MEAS_Q) && NC == 3 && G.ncirc > 0) A[MCG_A_Q] += F[8] / REF_PI / 4;
+    if ((flags & MCG_MEAS_RNORM) && G.nr > 0) {
+      for (int k = 0; k < NC; ++k) {
+        A[MCG_A_SIR + k] += fabs(F[16 + k] / F[24]);
+        A[MCG_A_SJR + k] += fabs(F[19 + k] / F[25]);
+      }
+      A[MCG_A_SIJR] += F[22] / F[26];
+    }
+    // groups
+    if (ng > 0) {
+      double* GA = B.gacc + (size_t)row * ((ng + 2) * (ng + 1));
+      double Gv[17][3];
+      for (int a = 0; a < ng; ++a)
+        for (int k = 0; k < 3; ++k) Gv[a][k] = F[32 + 3 * a + k];
+      for (int k = 0; k < 3; ++k) Gv[ng][k] = Mv[k] / nLat;
+      for (int a = 0; a <= ng; ++a)
+        for (int b2 = 0; b2 <= ng; ++b2) {
+          const double d = Gv[a][0] * Gv[b2][0] + Gv[a][1] * Gv[b2][1] + Gv[a][2] * Gv[b2][2];
+          GA[a * (ng + 1) + b2] += d;
+          if (a == b2) GA[(ng + 1) * (ng + 1) + a] += d * d;
+        }
+    }
+  }
+  if ((flags & MCG_MEAS_RNORM) && G.nr > 0) {
+    const double er = (F[27] / 2 + F[28]) / G.nr;
+    A[MCG_A_ER] += er;
+    A[MCG_A_ER2] += er * er;
+  }
+  A[MCG_A_E] += e;
+  A[MCG_A_E2] += e * e;
+  A[MCG_A_M] += Mf;
+  A[MCG_A_M2] += Mf * Mf;
+  A[MCG_A_M4] += Mf * Mf * Mf * Mf;
+  A[MCG_A_MM] += A[MCG_A_MPREV] * Mf;
+  A[MCG_A_MPREV] = Mf;
+  A[MCG_A_COUNT] += 1.0;
+  A[MCG_A_ABSM] += fabs(Mf);
+  // bins for error bars
+  int64_t nf = B.nframes > 0 ? B.nframes : 1;
+  int64_t bin = (frame * (int64_t)B.nbins) / nf;
+  if (bin >= B.nbins) bin = B.nbins - 1;
+  if (bin < 0) bin = 0;
+  double* Bq = B.bins + ((size_t)row * B.nbins + bin) * MCG_NBINQ;
+  Bq[0] += e;
+  Bq[1] += e * e;
+  Bq[2] += Mf;
+  Bq[3] += Mf * Mf;
+  Bq[4] += Mf * Mf * Mf * Mf;
+}
+
+// full FP64 recomputation of E and M of one replica (group-parallel)
+template <int NC, typename ST>
+__device__ void recompute(const Spins<ST>& sp, const mcg_graph& G, const float* coup, const float* clsdat, float h,
+                          double* s_red, int g, int lt, int TPR, double* out4) {
+  double v[RK];
+  for (int k = 0; k < RK; ++k) v[k] = 0.0;
+  for (int i = lt; i < G.n; i += TPR) {
+    v[0] += (double)site_energy<NC, ST>(sp, i, G.link, G.zmax, coup, G.cls, clsdat, h);
+    float s[3];
+    spin_vec<NC, ST>(sp, i, G.cls, clsdat, s);
+    v[1] += s[0];
+    v[2] += s[1];
+    v[3] += s[2];
+  }
+  group_sum8(v, s_red, g, TPR);
+  if (lt == 0)
+    for (int k = 0; k < 4; ++k) out4[k] = v[k];
+}
+
+// ---------------------------------------------------------------------------
+// k_lds: persistent LDS-resident sweeps
+// ---------------------------------------------------------------------------
+struct WorkItem {
+  int32_t graph, nrep, off, pad;
+};
+
+struct SweepArgs {
+  int64_t k_begin;      // run-relative index of the first sweep of this launch
+  int64_t philox_base;  // added to k for the Philox counter
+  int32_t nsweeps;
+  int32_t nthermal;     // run-relative: sweeps < nthermal are thermal
+  int32_t spf;          // sweeps per frame
+  int32_t frame0;       // index of the first frame of the run
+};
+
+static constexpr int FRM = 96;  // frame scratch doubles per group
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+__host__ __device__ inline size_t lds_fixed_bytes(int ncoup, int ncls, int nwaves, int R) {
+  return align16((size_t)(ncoup * 9 + ncls * 4) * 4) + align16((size_t)nwaves * RK * 8) +
+         align16((size_t)R * (MCG_NSTATE + FRM) * 8);
+}
+
+template <int NC, typename ST>
+__global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __restrict__ items,
+                                              const int32_t* __restrict__ item_reps, SweepArgs a) {
+  const WorkItem w = items[blockIdx.x];
+  const mcg_graph G = B.graphs[w.graph];
+  const int R = w.nrep;
+  const int TPR = blockDim.x / R;
+  const int g = threadIdx.x / TPR;
+  const int lt = threadIdx.x - g * TPR;
+  const int nwaves = blockDim.x / 64;
+
+  extern __shared__ __align__(16) unsigned char smem[];
+  float* s_coup = (float*)smem;
+  float* s_cls = s_coup + G.ncoup * 9;
+  size_t off = align16((size_t)(G.ncoup * 9 + G.ncls * 4) * 4);
+  double* s_red = (double*)(smem + off);
+  off += align16((size_t)nwaves * RK * 8);
+  double* s_st = (double*)(smem + off);  // R * NSTATE
+  double* s_frm = s_st + R * MCG_NSTATE;  // R * FRM
+  off += align16((size_t)R * (MCG_NSTATE + FRM) * 8);
+  ST* s_spin = (ST*)(smem + off);
+
+  for (int t = threadIdx.x; t < G.ncoup * 9; t += blockDim.x) s_coup[t] = G.coup[t];
+  for (int t = threadIdx.x; t < G.ncls * 4; t += blockDim.x) s_cls[t] = G.clsdat[t];
+
+  const int rep = item_reps[w.off + g];
+  const mcg_replica rp = B.reps[rep];
+  ST* gsp = (ST*)B.spins + rp.spin_off;
+  const Spins<ST> sp{s_spin + (size_t)g * NC * G.n, G.n};
+  for (int t = lt; t < NC * G.n; t += TPR) sp.p[t] = gsp[t];
+  double* st = s_st + g * MCG_NSTATE;
+  if (lt < MCG_NSTATE) st[lt] = B.state[(size_t)rep * MCG_NSTATE + lt];
+  __syncthreads();
+
+  const uint32_t key0 = B.seed_lo, key1 = B.seed_hi ^ rp.serial;
+  const int row = rp.row0 + B.rung[rep];
+  const float beta = B.row_beta[row], h = B.row_h[row];
+  double dE = 0.0, dM[3] = {0.0, 0.0, 0.0};
+  FrameCtx C{&G, s_coup, s_cls, s_red, s_frm + g * FRM, g, lt, TPR};
+
+  for (int s = 0; s < a.nsweeps; ++s) {
+    const int64_t k = a.k_begin + s;
+    const uint64_t kk = (uint64_t)(a.philox_base + k);
+    for (int c = 0; c < G.ncol; ++c) {
+      const int b = G.col_ptr[c], e = G.col_ptr[c + 1];
+      for (int j = b + lt; j < e; j += TPR)
+        metro_site<NC, ST>(sp, G.col_site[j], G.link, G.zmax, s_coup, G.cls, s_cls, beta, h, kk, key0, key1, dE, dM);
+      __syncthreads();
+    }
+    const bool meas = k >= a.nthermal && ((k - a.nthermal + 1) % a.spf) == 0;
+    const bool therm_end = (k + 1 == a.nthermal);
+    const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
+    if (meas || therm_end || rec || s == a.nsweeps - 1) {
+      double v[RK] = {dE, dM[0], dM[1], dM[2], 0, 0, 0, 0};
+      group_sum8(v, s_red, g, TPR);
+      if (lt == 0) {
+        st[MCG_S_E] += v[0];
+        st[MCG_S_M] += v[1];
+        st[MCG_S_M + 1] += v[2];
+        st[MCG_S_M + 2] += v[3];
+      }
+      dE = 0.0;
+      dM[0] = dM[1] = dM[2] = 0.0;
+      if (rec) {
+        double r4[4];
+        recompute<NC, ST>(sp, G, s_coup, s_cls, h, s_red, g, lt, TPR, r4);
+        if (lt == 0) {
+          const double d = fabs(st[MCG_S_E] - r4[0]);
+          if (d > st[MCG_S_DRIFT]) st[MCG_S_DRIFT] = d;
+          st[MCG_S_E] = r4[0];
+          st[MCG_S_M] = r4[1];
+          st[MCG_S_M + 1] = r4[2];
+          st[MCG_S_M + 2] = r4[3];
+        }
+      }
+      if (therm_end && lt == 0) st[MCG_S_EREF] = st[MCG_S_E];
+      __syncthreads();
+      if (meas) measure_frame<NC, ST>(sp, C, B, st, row, (k - a.nthermal + 1) / a.spf - 1 + a.frame0, key0, key1, h);
+      __syncthreads();
+    }
+  }
+  for (int t = lt; t < NC * G.n; t += TPR) gsp[t] = sp.p[t];
+  if (lt < MCG_NSTATE) B.state[(size_t)rep * MCG_NSTATE + lt] = st[lt];
+}
+
+// ---------------------------------------------------------------------------
+// k_pass: one colour pass, spins in global memory.  grid = (chunks, nrep)
+// ---------------------------------------------------------------------------
+template <int NC, typename ST>
+__global__ __launch_bounds__(256) void k_pass(mcg_batch B, const int32_t* __restrict__ rep_list, int colour,
+                                              int64_t sweep) {
+  const int rep = rep_list[blockIdx.y];
+  const mcg_replica rp = B.reps[rep];
+  const mcg_graph G = B.graphs[rp.graph];
+  double dE = 0.0, dM[3] = {0.0, 0.0, 0.0};
+  if (colour < G.ncol) {
+    const int b = G.col_ptr[colour], e = G.col_ptr[colour + 1];
+    const int j = b + blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < e) {
+      const int row = rp.row0 + B.rung[rep];
+      const Spins<ST> sp{(ST*)B.spins + rp.spin_off, G.n};
+      metro_site<NC, ST>(sp, G.col_site[j], G.link, G.zmax, G.coup, G.cls, G.clsdat, B.row_beta[row], B.row_h[row],
+                         (uint64_t)sweep, B.seed_lo, B.seed_hi ^ rp.serial, dE, dM);
+    }
+  }
+  // block reduction, one FP64 atomic per value per block
+  __shared__ double s_red[4 * RK];
+  double v[4] = {dE, dM[0], dM[1], dM[2]};
+  for (int k = 0; k < 4; ++k) {
+    double x = v[k];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    v[k] = x;
+  }
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 4; ++k) s_red[(threadIdx.x >> 6) * 4 + k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[4] = {0, 0, 0, 0};
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q)
+      for (int k = 0; k < 4; ++k) t[k] += s_red[q * 4 + k];
+    if (t[0] != 0.0 || t[1] != 0.0 || t[2] != 0.0 || t[3] != 0.0) {
+      double* st = B.state + (size_t)rep * MCG_NSTATE;
+      atomicAdd(st + MCG_S_E, t[0]);
+      atomicAdd(st + MCG_S_M, t[1]);
+      atomicAdd(st + MCG_S_M + 1, t[2]);
+      atomicAdd(st + MCG_S_M + 2, t[3]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_measure: one block per replica.  mode bits: 1 recompute, 2 set E_ref,
+// 4 measure frame, 8 reset drift
+// ---------------------------------------------------------------------------
+template <int NC, typename ST>
+__global__ __launch_bounds__(256) void k_measure(mcg_batch B, const int32_t* __restrict__ rep_list, int mode,
+                                                 int64_t frame) {
+  const int rep = rep_list[blockIdx.x];
+  const mcg_replica rp = B.reps[rep];
+  const mcg_graph G = B.graphs[rp.graph];
+  __shared__ double s_red[4 * RK];
+  __shared__ double s_frm[FRM];
+  __shared__ double st[MCG_NSTATE];
+  const Spins<ST> sp{(ST*)B.spins + rp.spin_off, G.n};
+  const int row = rp.row0 + B.rung[rep];
+  const float h = B.row_h[row];
+  if (threadIdx.x < MCG_NSTATE) st[threadIdx.x] = B.state[(size_t)rep * MCG_NSTATE + threadIdx.x];
+  __syncthreads();
+  if (mode & 1) {
+    double r4[4];
+    recompute<NC, ST>(sp, G, G.coup, G.clsdat, h, s_red, 0, threadIdx.x, blockDim.x, r4);
+    if (threadIdx.x == 0) {
+      if (!(mode & 8)) {
+        const double d = fabs(st[MCG_S_E] - r4[0]);
+        if (d > st[MCG_S_DRIFT]) st[MCG_S_DRIFT] = d;
+      }
+      st[MCG_S_E] = r4[0];
+      st[MCG_S_M] = r4[1];
+      st[MCG_S_M + 1] = r4[2];
+      st[MCG_S_M + 2] = r4[3];
+    }
+  }
+  if ((mode & 2) && threadIdx.x == 0) st[MCG_S_EREF] = st[MCG_S_E];
+  __syncthreads();
+  if (mode & 4) {
+    FrameCtx C{&G, G.coup, G.clsdat, s_red, s_frm, 0, (int)threadIdx.x, (int)blockDim.x};
+    measure_frame<NC, ST>(sp, C, B, st, row, frame, B.seed_lo, B.seed_hi ^ rp.serial, h);
+  }
+  __syncthreads();
+  if (threadIdx.x < MCG_NSTATE) B.state[(size_t)rep * MCG_NSTATE + threadIdx.x] = st[threadIdx.x];
+}
+
+// ---------------------------------------------------------------------------
+// k_pt: tempering swaps, one thread per ladder.  Pairs (t, t+1) with
+// t = parity, parity+2, ...; parity alternates with the swap count.
+// ---------------------------------------------------------------------------
+__global__ void k_pt(mcg_batch B, const int32_t* __restrict__ ladder_row0, const int32_t* __restrict__ ladder_n,
+                     int nladder, int64_t swap_index) {
+  const int L = blockIdx.x * blockDim.x + threadIdx.x;
+  if (L >= nladder) return;
+  const int r0 = ladder_row0[L], nr = ladder_n[L];
+  const int parity = (int)(swap_index & 1);
+  for (int t = parity; t + 1 < nr; t += 2) {
+    const int ra = B.who[r0 + t], rb = B.who[r0 + t + 1];
+    const double Ea = B.state[(size_t)ra * MCG_NSTATE + MCG_S_E], Eb = B.state[(size_t)rb * MCG_NSTATE + MCG_S_E];
+    const double ba = B.row_beta[r0 + t], bb = B.row_beta[r0 + t + 1];
+    const double d = (ba - bb) * (Ea - Eb);
+    bool acc = d >= 0.0;
+    if (!acc) {
+      const u32x4 x = philox4x32_10(
+          u32x4{(uint32_t)t, (uint32_t)swap_index, (uint32_t)((uint64_t)swap_index >> 32), TAG_PT}, B.seed_lo,
+          B.seed_hi ^ (0x80000000u | (uint32_t)L));
+      acc = u24(x.x) < exp_neg((float)d);
+    }
+    if (acc) {
+      B.who[r0 + t] = rb;
+      B.who[r0 + t + 1] = ra;
+      B.rung[ra] = t + 1;
+      B.rung[rb] = t;
+      B.pt_acc[r0 + t] += 1;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_wolff: one block per replica, ncl clusters, frontier BFS in global memory
+// (heisenbergLib.c:310-439, xyLib.c:256-380, isingLib.c:165-236)
+// ---------------------------------------------------------------------------
+template <int NC, typename ST>
+__global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __restrict__ rep_list, int64_t cl0,
+                                                int ncl) {
+  const int rep = rep_list[blockIdx.x];
+  const mcg_replica rp = B.reps[rep];
+  const mcg_graph G = B.graphs[rp.graph];
+  const Spins<ST> sp{(ST*)B.spins + rp.spin_off, G.n};
+  uint32_t* stamp = B.wstamp + rp.site_off;
+  int32_t* mem = B.wlist + rp.site_off;
+  const int row = rp.row0 + B.rung[rep];
+  const float beta = B.row_beta[row], h = B.row_h[row];
+  const uint32_t key0 = B.seed_lo, key1 = B.seed_hi ^ rp.serial;
+  __shared__ int s_cnt, s_lo, s_hi, s_seed;
+  __shared__ float s_r[3];
+  __shared__ double s_red[16];
+  __shared__ int s_acc;
+  const int tid = threadIdx.x;
+
+  for (int c = 0; c < ncl; ++c) {
+    const uint64_t cl = (uint64_t)(cl0 + c);
+    const uint32_t cur = (uint32_t)(cl & 0x7FFFFFFFu) * 2u + 1u;  // odd: never 0
+    if (tid == 0) {
+      const u32x4 x = philox4x32_10(u32x4{0u, (uint32_t)cl, (uint32_t)(cl >> 32), TAG_WOLFF}, key0, key1);
+      const int seed = (int)(((uint64_t)x.x * (uint64_t)G.n) >> 32);
+      float rx = 0, ry = 0, rz = 0;
+      if constexpr (NC == 3) {
+        rz = fmaf(u24(x.y), 2.0f, -1.0f);
+        const float rho = sqrtf(fmaxf(fmaf(-rz, rz, 1.0f), 0.0f));
+        float cc, sn;
+        sincos_turn(x.z, &cc, &sn);
+        rx = rho * cc;
+        ry = rho * sn;
+      } else if constexpr (NC == 2) {
+        sincos_turn(x.y, &rx, &ry);
+      }
+      s_r[0] = rx; s_r[1] = ry; s_r[2] = rz;
+      s_seed = seed;
+      atomicExch(stamp + seed, cur);
+      mem[0] = seed;
+      s_cnt = 1;
+      s_lo = 0;
+      s_hi = 1;
+      // cluster accept draw: x.w
+      s_acc = (int)(x.w >> 8);
+    }
+    __syncthreads();
+    const float r0 = s_r[0], r1 = s_r[1], r2 = s_r[2];
+    // level-synchronous growth
+    while (true) {
+      const int lo = s_lo, hi = s_hi;
+      if (lo >= hi) break;
+      for (int f = lo + tid; f < hi; f += blockDim.x) {
+        const int o = __hip_atomic_load(mem + f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        float so[3];
+        spin_vec<NC, ST>(sp, o, G.cls, G.clsdat, so);
+        const float sdo = -(so[0] * r0 + so[1] * r1 + so[2] * r2);
+        for (int l = 0; l < G.zmax; ++l) {
+          const uint32_t wd = G.link[(size_t)o * G.zmax + l];
+          const int t = (int)(wd & 0xFFFFFFu);
+          if (t == o) continue;
+          if (__hip_atomic_load(stamp + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == cur) continue;
+          const float* J = G.coup + (wd >> 24) * 9;
+          float corr;
+          if constexpr (NC == 1) {
+            corr = (J[0] * sp.get(t, 0)) * so[0] * beta;  // J*S_t*sigma_t * s_o
+            corr = 2.0f * corr;                            // add if J s s < 0, p = 1-exp(2 corr)
+          } else {
+            float st3[3];
+            spin_vec<NC, ST>(sp, t, G.cls, G.clsdat, st3);
+            const float sdt = -(st3[0] * r0 + st3[1] * r1 + st3[2] * r2);
+            const float rJr = r0 * (J[0] * r0 + J[1] * r1 + J[2] * r2) + r1 * (J[3] * r0 + J[4] * r1 + J[5] * r2) +
+                              r2 * (J[6] * r0 + J[7] * r1 + J[8] * r2);
+            corr = 2.0f * sdo * sdt * rJr * beta;
+          }
+          if (corr < 0.0f) {
+            const u32x4 x = philox4x32_10(
+                u32x4{(uint32_t)(o * G.zmax + l), (uint32_t)cl, (uint32_t)(cl >> 32), TAG_BOND}, key0, key1);
+            if (1.0f - exp_neg(corr) > u24(x.x)) {
+              const uint32_t old = atomicExch(stamp + t, cur);
+              if (old != cur) {
+                const int pos = atomicAdd(&s_cnt, 1);
+                __hip_atomic_store(mem + pos, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              }
+            }
+          }
+        }
+      }
+      __syncthreads();
+      if (tid == 0) {
+        s_lo = hi;
+        s_hi = s_cnt;
+      }
+      __syncthreads();
+    }
+    const int cnt = s_cnt;
+    // cluster acceptance energy (field / anisotropy correction)
+    double d = 0.0;
+    for (int f = tid; f < cnt; f += blockDim.x) {
+      const int b = __hip_atomic_load(mem + f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      float sb[3];
+      spin_vec<NC, ST>(sp, b, G.cls, G.clsdat, sb);
+      if constexpr (NC == 1) {
+        d += (double)(2.0f * h * sb[0]);
+      } else {
+        const float sdb = -(sb[0] * r0 + sb[1] * r1 + sb[2] * r2);
+        const float pb[3] = {sb[0] + sdb * r0, sb[1] + sdb * r1, sb[2] + sdb * r2};
+        for (int l = 0; l < G.zmax; ++l) {
+          const uint32_t wd = G.link[(size_t)b * G.zmax + l];
+          const int t = (int)(wd & 0xFFFFFFu);
+          const float* J = G.coup + (wd >> 24) * 9;
+          if ((wd >> 24) == 0u) continue;
+          float stt[3];
+          spin_vec<NC, ST>(sp, t, G.cls, G.clsdat, stt);
+          const float sdt = -(stt[0] * r0 + stt[1] * r1 + stt[2] * r2);
+          const float pt[3] = {stt[0] + sdt * r0, stt[1] + sdt * r1, stt[2] + sdt * r2};
+          // r^T J p_t
+          const float src = sdb * (r0 * (J[0] * pt[0] + J[1] * pt[1] + J[2] * pt[2]) +
+                                   r1 * (J[3] * pt[0] + J[4] * pt[1] + J[5] * pt[2]) +
+                                   r2 * (J[6] * pt[0] + J[7] * pt[1] + J[8] * pt[2]));
+          d += (double)src;
+          if (__hip_atomic_load(stamp + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == cur)
+            d += (double)(sdt * (pb[0] * (J[0] * r0 + J[1] * r1 + J[2] * r2) +
+                                 pb[1] * (J[3] * r0 + J[4] * r1 + J[5] * r2) +
+                                 pb[2] * (J[6] * r0 + J[7] * r1 + J[8] * r2)));
+          else
+            d += (double)src;
+        }
+        // single-ion term with the half step s + sdn*r (heisenbergLib.c:418)
+        const int ci = G.cls[b];
+        const float* Dd = G.clsdat + ci * 4 + 1;
+        const float tr[3] = {sdb * r0, sdb * r1, sdb * r2};
+        float o = 0.0f;
+        for (int k = 0; k < NC; ++k) {
+          const float q = sb[k] + tr[k];
+          o += Dd[k] * (q * q - sb[k] * sb[k]);
+        }
+        o -= h * tr[NC == 3 ? 2 : 0];
+        d += (double)o;
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o, 64);
+    if ((tid & 63) == 0) s_red[tid >> 6] = d;
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+      for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += s_red[q];
+      const float db = (float)(t * (double)beta);
+      const float u = (float)s_acc * 0x1p-24f;
+      s_acc = (db <= 0.0f || u < exp_neg(-db)) ? 1 : 0;
+    }
+    __syncthreads();
+    if (s_acc) {
+      for (int f = tid; f < cnt; f += blockDim.x) {
+        const int b = __hip_atomic_load(mem + f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if constexpr (NC == 1) {
+          sp.set(b, 0, -sp.get(b, 0));
+        } else {
+          float sb[3];
+          spin_vec<NC, ST>(sp, b, G.cls, G.clsdat, sb);
+          const float a2 = -2.0f * (sb[0] * r0 + sb[1] * r1 + sb[2] * r2);
+          float n0 = sb[0] + a2 * r0, n1 = sb[1] + a2 * r1, n2 = sb[2] + a2 * r2;
+          float nn = n0 * n0 + n1 * n1 + n2 * n2;
+          const float fct = fabsf(G.clsdat[G.cls[b] * 4]) / sqrtf(nn);
+          sp.set(b, 0, n0 * fct);
+          sp.set(b, 1, n1 * fct);
+          if constexpr (NC == 3) sp.set(b, 2, n2 * fct);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// small utility kernels
+// ---------------------------------------------------------------------------
+__global__ void k_fill_f64(double* p, size_t n, double v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_i32(int32_t* p, size_t n, int32_t v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ===========================================================================
+// host side
+// ===========================================================================
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int upload(const void* src, size_t n, hipStream_t s) {
+    if (n == 0) return 0;
+    if (!p || bytes < n) {
+      if (p) (void)hipFree(p);
+      p = nullptr;
+      if (hipMalloc(&p, n) != hipSuccess) return fail("hipMalloc(%zu) failed", n);
+      bytes = n;
+    }
+    if (hipMemcpyAsync(p, src, n, hipMemcpyHostToDevice, s) != hipSuccess) return fail("hipMemcpyAsync failed");
+    return 0;
+  }
+};
+
+int store_size(int store) { return store == MCG_STORE_F32 ? 4 : store == MCG_STORE_F16 ? 2 : 1; }
+
+int check_batch(const mcg_batch* B) {
+  if (!B) return fail("null batch");
+  if (B->nc < 1 || B->nc > 3) return fail("nc must be 1, 2 or 3 (got %d)", B->nc);
+  if (B->nc == 1 && B->store != MCG_STORE_I8) return fail("Ising batches use int8 storage");
+  if (B->nc > 1 && B->store == MCG_STORE_I8) return fail("int8 storage is for Ising only");
+  if (!B->graphs_host || !B->reps_host) return fail("batch lacks host mirrors of graphs/reps");
+  if (B->nbins < 1) return fail("nbins must be >= 1");
+  for (int gi = 0; gi < B->ngraph; ++gi) {
+    const mcg_graph& G = B->graphs_host[gi];
+    if (G.n <= 0 || G.n > (1 << 24)) return fail("graph %d: n=%d outside 1..2^24", gi, G.n);
+    if (G.ncoup < 1 || G.ncoup > 256) return fail("graph %d: ncoup=%d outside 1..256", gi, G.ncoup);
+    if (G.ncls < 1 || G.ncls > 256) return fail("graph %d: ncls=%d outside 1..256", gi, G.ncls);
+    if (G.ncol < 1) return fail("graph %d: no colours", gi);
+    if ((B->meas & MCG_MEAS_GROUPS) && G.ngroup > 16) return fail("graph %d: more than 16 orbital groups", gi);
+  }
+  for (int r = 0; r < B->nrep; ++r) {
+    const mcg_replica& R = B->reps_host[r];
+    if (R.graph < 0 || R.graph >= B->ngraph) return fail("replica %d: bad graph index", r);
+    if (R.row0 < 0 || R.row0 + R.nrung > B->nrow) return fail("replica %d: rows outside the batch", r);
+  }
+  return 0;
+}
+
+// launch-time resources kept for one call
+struct RunCtx {
+  DevBuf items, item_reps, all_reps, stream_reps, lad_row0, lad_n;
+  std::vector<WorkItem> hitems;
+  std::vector<int32_t> hitem_reps, hstream, hall;
+  int block = 0;
+  size_t lds_max = 0;
+  std::vector<int32_t> hl_row0, hl_n;
+};
+
+int device_lds_max(size_t* out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  int v = 0;
+  HIPCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  *out = (size_t)v;
+  return 0;
+}
+
+template <int NC, typename ST>
+int set_lds_attr(size_t bytes) {
+  hipError_t e = hipFuncSetAttribute((const void*)k_lds<NC, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  (void)e;
+  (void)hipGetLastError();
+  return 0;
+}
+
+// plan the LDS work items: replicas grouped by graph, R per block
+int plan_lds(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool force_stream) {
+  const int ss = store_size(B->store);
+  X.hitems.clear();
+  X.hitem_reps.clear();
+  X.hstream.clear();
+  X.hall.clear();
+  for (int r = 0; r < B->nrep; ++r) X.hall.push_back(r);
+  const int block = P->block > 0 ? P->block : 256;
+  if (block % 64 || block < 64 || block > 1024) return fail("block must be a multiple of 64 in [64,1024]");
+  X.block = block;
+  const int nwaves = block / 64;
+  // group replicas by graph, preserving order
+  std::vector<std::vector<int>> by(B->ngraph);
+  for (int r = 0; r < B->nrep; ++r) by[B->reps_host[r].graph].push_back(r);
+  for (int gi = 0; gi < B->ngraph; ++gi) {
+    const mcg_graph& G = B->graphs_host[gi];
+    const std::vector<int>& rs = by[gi];
+    if (rs.empty()) continue;
+    const size_t per = align16((size_t)B->nc * G.n * ss);
+    int Rmax = 0;
+    if (!force_stream) {
+      for (int R = nwaves; R >= 1; --R) {
+        if (nwaves % R) continue;
+        if (P->reps_per_block > 0 && R > P->reps_per_block) continue;
+        if (lds_fixed_bytes(G.ncoup, G.ncls, nwaves, R) + R * per <= X.lds_max) {
+          Rmax = R;
+          break;
+        }
+      }
+    }
+    if (Rmax == 0) {
+      for (int r : rs) X.hstream.push_back(r);
+      continue;
+    }
+    const int R = Rmax;
+    for (size_t i = 0; i < rs.size(); i += R) {
+      const int cnt = (int)std::min<size_t>(R, rs.size() - i);
+      // a block's groups must tile the block: pad by repeating is not allowed,
+      // so shrink R for the tail item to a divisor of nwaves
+      int Rt = cnt;
+      while (nwaves % Rt) --Rt;
+      for (size_t q = 0; q < (size_t)cnt; q += Rt) {
+        const int c2 = (int)std::min<size_t>(Rt, cnt - q);
+        int Rq = c2;
+        while (nwaves % Rq) --Rq;
+        // c2 == Rq always holds here because Rt divides nwaves and c2 <= Rt
+        X.hitems.push_back(WorkItem{gi, Rq, (int)X.hitem_reps.size(), 0});
+        for (int k = 0; k < Rq; ++k) X.hitem_reps.push_back(rs[i + q + k]);
+        if (Rq < c2) return fail("internal: work item split");
+      }
+    }
+  }
+  return 0;
+}
+
+template <int NC, typename ST>
+size_t lds_bytes_for(const mcg_batch* B, const RunCtx& X) {
+  size_t mx = 0;
+  const int nwaves = X.block / 64;
+  for (const WorkItem& w : X.hitems) {
+    const mcg_graph& G = B->graphs_host[w.graph];
+    const size_t b = lds_fixed_bytes(G.ncoup, G.ncls, nwaves, w.nrep) + w.nrep * align16((size_t)NC * G.n * sizeof(ST));
+    mx = std::max(mx, b);
+  }
+  return mx;
+}
+
+template <int NC, typename ST>
+int launch_lds(const mcg_batch* B, RunCtx& X, const SweepArgs& a, hipStream_t s) {
+  if (X.hitems.empty()) return 0;
+  const size_t bytes = lds_bytes_for<NC, ST>(B, X);
+  set_lds_attr<NC, ST>(bytes);
+  hipLaunchKernelGGL((k_lds<NC, ST>), dim3((unsigned)X.hitems.size()), dim3(X.block), bytes, s, *B,
+                     (const WorkItem*)X.items.p, (const int32_t*)X.item_reps.p, a);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int max_colour_count(const mcg_batch* B, const std::vector<int32_t>& reps, int* ncol_max) {
+  int mx = 0, nc = 0;
+  for (int r : reps) {
+    const mcg_graph& G = B->graphs_host[B->reps_host[r].graph];
+    mx = std::max(mx, G.n);
+    nc = std::max(nc, G.ncol);
+  }
+  *ncol_max = nc;
+  return mx;
+}
+
+template <int NC, typename ST>
+int launch_pass(const mcg_batch* B, const int32_t* dreps, int nreps, int maxn, int colour, int64_t sweep,
+                hipStream_t s) {
+  if (nreps == 0) return 0;
+  const unsigned chunks = (unsigned)((maxn + 255) / 256);
+  hipLaunchKernelGGL((k_pass<NC, ST>), dim3(chunks, (unsigned)nreps), dim3(256), 0, s, *B, dreps, colour, sweep);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <int NC, typename ST>
+int launch_measure(const mcg_batch* B, const int32_t* dreps, int nreps, int mode, int64_t frame, hipStream_t s) {
+  if (nreps == 0) return 0;
+  hipLaunchKernelGGL((k_measure<NC, ST>), dim3((unsigned)nreps), dim3(256), 0, s, *B, dreps, mode, frame);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int launch_fill_f64(double* p, size_t n, double v, hipStream_t s) {
+  if (!p || n == 0) return 0;
+  const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_f64, dim3(g), dim3(256), 0, s, p, n, v);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+int launch_fill_i32(int32_t* p, size_t n, int32_t v, hipStream_t s) {
+  if (!p || n == 0) return 0;
+  const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_i32, dim3(g), dim3(256), 0, s, p, n, v);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// planned sweep schedule for the streaming / plain paths
+template <int NC, typename ST>
+int stream_sweeps(const mcg_batch* B, const mcg_plan* P, RunCtx& X, int64_t k0, int64_t nsw, int64_t nthermal,
+                  int spf, bool plain, hipStream_t s) {
+  if (X.hstream.empty()) return 0;
+  int ncolmax = 0;
+  const int maxn = max_colour_count(B, X.hstream, &ncolmax);
+  const int32_t* dreps = (const int32_t*)X.stream_reps.p;
+  const int nrs = (int)X.hstream.size();
+  for (int64_t q = 0; q < nsw; ++q) {
+    const int64_t k = k0 + q;
+    const int64_t kk = P->sweep0 + k;
+    for (int c = 0; c < ncolmax; ++c) {
+      if (plain) {
+        for (int i = 0; i < nrs; ++i)
+          if (launch_pass<NC, ST>(B, dreps + i, 1, maxn, c, kk, s)) return -1;
+      } else if (launch_pass<NC, ST>(B, dreps, nrs, maxn, c, kk, s)) {
+        return -1;
+      }
+    }
+    const bool meas = k >= nthermal && ((k - nthermal + 1) % spf) == 0;
+    const bool therm_end = (k + 1 == nthermal);
+    const bool rec = B->recompute_every > 0 && ((k + 1) % B->recompute_every) == 0;
+    int mode = (rec ? 1 : 0) | (therm_end ? 2 : 0) | (meas ? 4 : 0);
+    if (mode) {
+      const int64_t frame = meas ? (k - nthermal + 1) / spf - 1 + P->frame0 : -1;
+      if (plain) {
+        for (int i = 0; i < nrs; ++i)
+          if (launch_measure<NC, ST>(B, dreps + i, 1, mode, frame, s)) return -1;
+      } else if (launch_measure<NC, ST>(B, dreps, nrs, mode, frame, s)) {
+        return -1;
+      }
+    }
+  }
+  return 0;
+}
+
+template <int NC, typename ST>
+int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
+  RunCtx X;
+  if (device_lds_max(&X.lds_max)) return -1;
+  const bool force_stream = P->kernel == 2 || P->kernel == 3;
+  const bool plain = P->kernel == 3;
+  if (plan_lds(B, P, X, force_stream)) return -1;
+  if (P->kernel == 1 && !X.hstream.empty())
+    return fail("LDS-resident kernel requested but %zu replica(s) do not fit in LDS", X.hstream.size());
+  if (X.items.upload(X.hitems.data(), X.hitems.size() * sizeof(WorkItem), s)) return -1;
+  if (X.item_reps.upload(X.hitem_reps.data(), X.hitem_reps.size() * 4, s)) return -1;
+  if (X.stream_reps.upload(X.hstream.data(), X.hstream.size() * 4, s)) return -1;
+  if (X.all_reps.upload(X.hall.data(), X.hall.size() * 4, s)) return -1;
+  // ladders
+  if (P->pt_every > 0) {
+    std::vector<int32_t> seen(B->nrow, 0);
+    for (int r = 0; r < B->nrep; ++r) {
+      const mcg_replica& R = B->reps_host[r];
+      if (R.ladder < 0 || R.nrung < 2 || seen[R.row0]) continue;
+      seen[R.row0] = 1;
+      X.hl_row0.push_back(R.row0);
+      X.hl_n.push_back(R.nrung);
+    }
+    if (X.lad_row0.upload(X.hl_row0.data(), X.hl_row0.size() * 4, s)) return -1;
+    if (X.lad_n.upload(X.hl_n.data(), X.hl_n.size() * 4, s)) return -1;
+  }
+  const int spf = P->sweeps_per_frame > 0 ? P->sweeps_per_frame : 1;
+  const int64_t nth = P->nthermal;
+  const int64_t total = nth + (int64_t)P->nsweep * spf;
+  const int nall = B->nrep;
+
+  if (P->algorithm == MCG_WOLFF) {
+    if (!B->wstamp || !B->wlist) return fail("Wolff needs wstamp/wlist scratch");
+    int64_t nmax = 0;
+    for (int gi = 0; gi < B->ngraph; ++gi) nmax = std::max<int64_t>(nmax, B->graphs_host[gi].n);
+    for (int64_t k = 0; k < total; ++k) {
+      const int64_t kk = P->sweep0 + k;
+      // clusters per sweep: plan value or n of the replica's graph (use max n)
+      const int ncl = P->clusters_per_sweep > 0 ? P->clusters_per_sweep : (int)nmax;
+      hipLaunchKernelGGL((k_wolff<NC, ST>), dim3((unsigned)nall), dim3(256), 0, s, *B,
+                         (const int32_t*)X.all_reps.p, kk * (int64_t)ncl, ncl);
+      HIPCHK(hipGetLastError());
+      const bool meas = k >= nth && ((k - nth + 1) % spf) == 0;
+      const bool therm_end = (k + 1 == nth);
+      const int mode = 1 | 8 | (therm_end ? 2 : 0) | (meas ? 4 : 0);
+      if (meas || therm_end || k + 1 == total) {
+        const int64_t frame = meas ? (k - nth + 1) / spf - 1 + P->frame0 : -1;
+        if (launch_measure<NC, ST>(B, (const int32_t*)X.all_reps.p, nall, mode, frame, s)) return -1;
+      }
+      if (P->pt_every > 0 && ((k + 1) % P->pt_every) == 0 && !X.hl_row0.empty()) {
+        if (launch_measure<NC, ST>(B, (const int32_t*)X.all_reps.p, nall, 1 | 8, -1, s)) return -1;
+        hipLaunchKernelGGL(k_pt, dim3((unsigned)((X.hl_row0.size() + 63) / 64)), dim3(64), 0, s, *B,
+                           (const int32_t*)X.lad_row0.p, (const int32_t*)X.lad_n.p, (int)X.hl_row0.size(),
+                           P->sweep0 + k);
+        HIPCHK(hipGetLastError());
+      }
+    }
+    return 0;
+  }
+
+  const int64_t chunk = P->pt_every > 0 ? P->pt_every : total;
+  for (int64_t k = 0; k < total;) {
+    const int64_t ns = std::min<int64_t>(chunk, total - k);
+    SweepArgs a{k, P->sweep0, (int32_t)ns, (int32_t)nth, spf, P->frame0};
+    if (launch_lds<NC, ST>(B, X, a, s)) return -1;
+    if (stream_sweeps<NC, ST>(B, P, X, k, ns, nth, spf, plain, s)) return -1;
+    k += ns;
+    if (P->pt_every > 0 && (k % P->pt_every) == 0 && !X.hl_row0.empty()) {
+      hipLaunchKernelGGL(k_pt, dim3((unsigned)((X.hl_row0.size() + 63) / 64)), dim3(64), 0, s, *B,
+                         (const int32_t*)X.lad_row0.p, (const int32_t*)X.lad_n.p, (int)X.hl_row0.size(),
+                         P->sweep0 + k - 1);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  return 0;
+}
+
+template <int NC, typename ST>
+int init_typed(const mcg_batch* B, hipStream_t s) {
+  std::vector<int32_t> all(B->nrep);
+  for (int r = 0; r < B->nrep; ++r) all[r] = r;
+  DevBuf d;
+  if (d.upload(all.data(), all.size() * 4, s)) return -1;
+  if (launch_fill_f64(B->state, (size_t)B->nrep * MCG_NSTATE, 0.0, s)) return -1;
+  if (launch_fill_f64(B->acc, (size_t)B->nrow * MCG_NACC, 0.0, s)) return -1;
+  if (launch_fill_f64(B->bins, (size_t)B->nrow * B->nbins * MCG_NBINQ, 0.0, s)) return -1;
+  if (B->gacc) {
+    int ng = 0;
+    for (int gi = 0; gi < B->ngraph; ++gi) ng = std::max(ng, B->graphs_host[gi].ngroup);
+    if (launch_fill_f64(B->gacc, (size_t)B->nrow * (ng + 2) * (ng + 1), 0.0, s)) return -1;
+  }
+  if (launch_fill_i32(B->pt_acc, (size_t)B->nrow, 0, s)) return -1;
+  if (launch_fill_i32(B->err, 1, 0, s)) return -1;
+  if (B->wstamp) {
+    size_t tot = 0;
+    for (int r = 0; r < B->nrep; ++r)
+      tot = std::max<size_t>(tot, (size_t)B->reps_host[r].site_off + B->graphs_host[B->reps_host[r].graph].n);
+    const unsigned g = (unsigned)std::min<size_t>((tot + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(256), 0, s, B->wstamp, tot, 0u);
+    HIPCHK(hipGetLastError());
+  }
+  if (launch_measure<NC, ST>(B, (const int32_t*)d.p, B->nrep, 1 | 2 | 8, -1, s)) return -1;
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+template <int NC, typename ST>
+int measure_typed(const mcg_batch* B, int64_t frame, hipStream_t s) {
+  std::vector<int32_t> all(B->nrep);
+  for (int r = 0; r < B->nrep; ++r) all[r] = r;
+  DevBuf d;
+  if (d.upload(all.data(), all.size() * 4, s)) return -1;
+  if (launch_measure<NC, ST>(B, (const int32_t*)d.p, B->nrep, 1 | 4, frame, s)) return -1;
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+}  // namespace
+
+#define DISPATCH(fn, ...)                                                                    \
+  do {                                                                                       \
+    if (B->nc == 1) return fn<1, int8_t>(__VA_ARGS__);                                       \
+    if (B->nc == 2 && B->store == MCG_STORE_F32) return fn<2, float>(__VA_ARGS__);           \
+    if (B->nc == 2 && B->store == MCG_STORE_F16) return fn<2, __half>(__VA_ARGS__);          \
+    if (B->nc == 3 && B->store == MCG_STORE_F32) return fn<3, float>(__VA_ARGS__);           \
+    if (B->nc == 3 && B->store == MCG_STORE_F16) return fn<3, __half>(__VA_ARGS__);          \
+    return fail("unsupported nc/store combination");                                         \
+  } while (0)
+
+extern "C" int mcg_init(const mcg_batch* B, void* stream) {
+  if (check_batch(B)) return -1;
+  hipStream_t s = (hipStream_t)stream;
+  DISPATCH(init_typed, B, s);
+}
+
+extern "C" int mcg_run(const mcg_batch* B, const mcg_plan* P, void* stream) {
+  if (check_batch(B)) return -1;
+  if (!P) return fail("null plan");
+  if (P->nthermal < 0 || P->nsweep < 0) return fail("negative sweep counts");
+  if (P->algorithm != MCG_METROPOLIS && P->algorithm != MCG_WOLFF) return fail("unknown algorithm %d", P->algorithm);
+  hipStream_t s = (hipStream_t)stream;
+  DISPATCH(run_typed, B, P, s);
+}
+
+extern "C" int mcg_sweeps(const mcg_batch* B, const mcg_plan* P, int64_t sweep0, int32_t nsweeps, void* stream) {
+  if (check_batch(B)) return -1;
+  if (!P) return fail("null plan");
+  mcg_plan q = *P;
+  q.nthermal = nsweeps;  // all sweeps unmeasured
+  q.nsweep = 0;
+  q.sweep0 = sweep0;
+  q.pt_every = 0;
+  hipStream_t s = (hipStream_t)stream;
+  DISPATCH(run_typed, B, &q, s);
+}
+
+extern "C" int mcg_measure(const mcg_batch* B, int64_t frame, void* stream) {
+  if (check_batch(B)) return -1;
+  hipStream_t s = (hipStream_t)stream;
+  DISPATCH(measure_typed, B, frame, s);
+}
+
+// greedy colouring in site order: each site takes the smallest colour not
+// used by an already-coloured neighbour (in- and out-links both count)
+extern "C" int mcg_colour_graph(int32_t n, int32_t zmax, const int32_t* nbr, int32_t* colour) {
+  if (n < 0 || zmax < 0 || (!nbr && n * zmax) || !colour) return fail("bad arguments to mcg_colour_graph");
+  std::vector<std::vector<int32_t>> adj(n);
+  for (int32_t i = 0; i < n; ++i)
+    for (int32_t l = 0; l < zmax; ++l) {
+      const int32_t j = nbr[(size_t)i * zmax + l];
+      if (j < 0 || j == i) continue;
+      if (j >= n) return fail("neighbour index %d out of range", j);
+      adj[i].push_back(j);
+      adj[j].push_back(i);
+    }
+  std::vector<int32_t> mark;
+  int ncol = 0;
+  for (int32_t i = 0; i < n; ++i) colour[i] = -1;
+  for (int32_t i = 0; i < n; ++i) {
+    mark.assign(ncol + 1, 0);
+    for (int32_t j : adj[i])
+      if (colour[j] >= 0 && colour[j] <= ncol) mark[colour[j]] = 1;
+    int c = 0;
+    while (c <= ncol && mark[c]) ++c;
+    colour[i] = c;
+    if (c + 1 > ncol) ncol = c + 1;
+  }
+  return ncol;
+}

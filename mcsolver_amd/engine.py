@@ -1,0 +1,361 @@
+"""Host side of the batched engine: compiles per-site link tables into the
+device layout of include/mcgpu.h, owns the device buffers (torch tensors on
+the ROCm device), and drives libmcgpu through the C ABI.
+
+Device layout (HBM), per lattice ("graph"), shared by all its replicas:
+  link[n*zmax]   uint32  neighbour | coupling-id << 24   (4 B / link)
+  coup[ncoup*9]  fp32    distinct 3x3 tensors, row 0 = zero (<= 256)
+  cls[n] uint8 + clsdat[ncls*4] fp32: (S, Dx, Dy, Dz) per site class
+  col_ptr/col_site: sites grouped by colour of a proper graph colouring
+Per replica: spins as structure-of-arrays, component c of replica r at
+spin_off + c*n (fp32, fp16 or int8 for Ising).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+# reference tensor order [xx,yy,zz,xy,xz,yz,yx,zx,zy] -> row-major 3x3
+_REF_TO_ROWMAJOR = np.array([0, 3, 4, 6, 1, 5, 7, 8, 2])
+
+
+def colour_graph(nbr: np.ndarray) -> np.ndarray:
+    """Proper colouring of the site graph (native greedy, libmcgpu)."""
+    nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+    n = nbr.shape[0]
+    z = nbr.shape[1] if nbr.ndim == 2 else 0
+    col = np.empty(n, np.int32)
+    L.check(L.lib().mcg_colour_graph(n, z, nbr.ctypes.data, col.ctypes.data), "mcg_colour_graph")
+    return col
+
+
+@dataclass
+class GraphTables:
+    """Host copy of one compiled graph (numpy), plus its device tensors."""
+    n: int
+    zmax: int
+    link: np.ndarray
+    coup: np.ndarray
+    cls: np.ndarray
+    clsdat: np.ndarray
+    col_ptr: np.ndarray
+    col_site: np.ndarray
+    pairs: np.ndarray
+    circ: np.ndarray
+    rorb: np.ndarray
+    rclu: np.ndarray
+    rlink: np.ndarray
+    groups: np.ndarray
+    gmax: int
+    rrank: np.ndarray
+    colour: np.ndarray
+    dev: Optional[dict] = None
+    struct: Optional[L.Graph] = None
+
+    @property
+    def ncol(self) -> int:
+        return len(self.col_ptr) - 1
+
+
+def compile_graph(nc: int, nbr: np.ndarray, J: np.ndarray, S: np.ndarray, D: np.ndarray,
+                  pairs: Optional[np.ndarray] = None, circuits: Optional[np.ndarray] = None,
+                  rOrb: Optional[np.ndarray] = None, rOrbCluster: Optional[np.ndarray] = None,
+                  rlink: Optional[np.ndarray] = None, groups: Optional[Sequence[Sequence[int]]] = None,
+                  diag_only: bool = False, colour: Optional[np.ndarray] = None) -> GraphTables:
+    """Per-site tables -> device tables.
+
+    nbr [n,zmax] (-1 padded), J [n,zmax,9] in reference order (Ising: J[...,0]),
+    S [n] signed spin length, D [n,3].  Couplings are unscaled (the engine
+    multiplies dE by beta) or pre-scaled with beta = 1, as the caller wishes.
+    """
+    nbr = np.asarray(nbr, np.int64)
+    n, zmax = nbr.shape
+    if n > (1 << 24):
+        raise ValueError("more than 2^24 sites per replica")
+    J = np.asarray(J, np.float64).reshape(n, zmax, -1)
+    S = np.asarray(S, np.float64).reshape(n)
+    D = np.asarray(D, np.float64).reshape(n, 3)
+    valid = nbr >= 0
+    if nbr.size and (nbr.max() >= n or nbr.min() < -1):
+        raise ValueError("neighbour index outside [-1, n)")
+    tgt = np.where(valid, nbr, np.arange(n)[:, None])
+    if nc == 1:
+        jr = np.where(valid, J[:, :, 0], 0.0)
+        folded = jr * S[tgt]
+        rows = np.zeros((n, zmax, 9), np.float32)
+        rows[:, :, 0] = folded
+        rows[:, :, 1] = jr
+    else:
+        jj = np.where(valid[:, :, None], J[:, :, :9], 0.0)
+        if diag_only:
+            jj = jj.copy()
+            jj[:, :, 3:] = 0.0
+        if nc == 2:  # XY reads xx, yy, xy, yx only (xyLib.c:150-153)
+            jj = jj.copy()
+            jj[:, :, [2, 4, 5, 7, 8]] = 0.0
+        rows = jj[:, :, _REF_TO_ROWMAJOR].astype(np.float32)
+    flat = rows.reshape(-1, 9)
+    allrows = np.concatenate([np.zeros((1, 9), np.float32), flat])
+    uniq, inv = np.unique(allrows, axis=0, return_inverse=True)
+    inv = inv.reshape(-1)
+    zero_id = inv[0]
+    # put the zero tensor at index 0
+    perm = np.arange(len(uniq))
+    perm[[0, zero_id]] = perm[[zero_id, 0]]
+    uniq = uniq[perm]
+    remap = np.empty_like(perm)
+    remap[perm] = np.arange(len(perm))
+    cid = remap[inv[1:]].reshape(n, zmax)
+    if len(uniq) > 256:
+        raise ValueError("more than 256 distinct coupling tensors (%d)" % len(uniq))
+    link = (tgt.astype(np.uint32) | (cid.astype(np.uint32) << 24)).astype(np.uint32)
+
+    cd = np.concatenate([S[:, None], D], 1).astype(np.float32)
+    cu, cinv = np.unique(cd, axis=0, return_inverse=True)
+    if len(cu) > 256:
+        raise ValueError("more than 256 distinct site classes")
+    cls = cinv.reshape(-1).astype(np.uint8)
+
+    if colour is None:
+        colour = colour_graph(np.where(valid, nbr, -1).astype(np.int32))
+    colour = np.asarray(colour, np.int32)
+    order = np.argsort(colour, kind="stable").astype(np.int32)
+    ncol = int(colour.max()) + 1 if n else 0
+    col_ptr = np.zeros(ncol + 1, np.int32)
+    col_ptr[1:] = np.cumsum(np.bincount(colour, minlength=ncol))
+
+    pairs = np.zeros((0, 2), np.int32) if pairs is None else np.asarray(pairs, np.int32).reshape(-1, 2)
+    circ = np.zeros((0, 3), np.int32) if circuits is None else np.asarray(circuits, np.int32).reshape(-1, 3)
+    rorb = np.zeros(0, np.int32) if rOrb is None else np.asarray(rOrb, np.int32).reshape(-1)
+    nr = len(rorb)
+    rclu = np.zeros((nr, 1), np.int32) if rOrbCluster is None or nr == 0 else \
+        np.asarray(rOrbCluster, np.int32).reshape(nr, -1)
+    rrank = -np.ones(n, np.int32)
+    rrank[rorb] = np.arange(nr, dtype=np.int32)
+    if nr and rlink is not None:
+        rl = np.asarray(rlink, np.int64).reshape(nr, -1)[:, :zmax]
+        if rl.shape[1] < zmax:
+            rl = np.concatenate([rl, -np.ones((nr, zmax - rl.shape[1]), np.int64)], 1)
+        # a block-spin link uses the coupling of the site's normal link l
+        # (heisenbergLib.c:269-279); the link must exist and point at a
+        # block-spin site, otherwise it contributes nothing
+        rc = cid[rorb]
+        ok = (rl >= 0) & (np.arange(zmax)[None, :] < (nbr[rorb] >= 0).sum(1)[:, None])
+        ok &= np.where(rl >= 0, rrank[np.clip(rl, 0, n - 1)] >= 0, False)
+        rt = np.where(ok, rl, rorb[:, None])
+        rc = np.where(ok, rc, 0)
+        rlk = (rt.astype(np.uint32) | (rc.astype(np.uint32) << 24)).astype(np.uint32)
+    else:
+        rlk = np.zeros((nr, zmax), np.uint32)
+    for name, arr in (("corrOrbPair", pairs), ("localCircuits", circ), ("rOrb", rorb), ("rOrbCluster", rclu)):
+        if arr.size and (arr.min() < 0 or arr.max() >= n):
+            raise ValueError("%s index outside [0, n)" % name)
+    groups = list(groups or [])
+    for gg in groups:
+        if len(gg) and (min(gg) < 0 or max(gg) >= n):
+            raise ValueError("orbital group index outside [0, n)")
+    if len(groups) > 16:
+        raise ValueError("at most 16 orbital groups are supported")
+    gmax = max([len(g) for g in groups] + [1])
+    garr = -np.ones((max(len(groups), 1), gmax), np.int32)
+    for k, g in enumerate(groups):
+        garr[k, :len(g)] = g
+    return GraphTables(n, zmax, link.reshape(-1), uniq.astype(np.float32).reshape(-1), cls, cu.reshape(-1),
+                       col_ptr, order, pairs, circ, rorb, rclu, rlk.reshape(-1), garr, gmax, rrank, colour)
+
+
+def _dev(a: np.ndarray, device) -> torch.Tensor:
+    a = np.ascontiguousarray(a)
+    if a.dtype == np.uint32:
+        a = a.view(np.int32)
+    if a.size == 0:
+        a = np.zeros(1, a.dtype)
+    return torch.from_numpy(a).to(device)
+
+
+def upload_graph(g: GraphTables, device) -> None:
+    if g.dev is not None:
+        return
+    d = {k: _dev(getattr(g, k), device) for k in ("link", "coup", "cls", "clsdat", "col_ptr", "col_site", "pairs",
+                                                  "circ", "rorb", "rclu", "rlink", "groups", "rrank")}
+    g.dev = d
+    s = L.Graph()
+    s.n, s.zmax, s.ncol = g.n, g.zmax, g.ncol
+    s.ncoup, s.ncls = len(g.coup) // 9, len(g.clsdat) // 4
+    s.nlat, s.ncirc, s.nr = len(g.pairs), len(g.circ), len(g.rorb)
+    s.nclu = g.rclu.shape[1] if len(g.rorb) else 0
+    s.ngroup = 0 if (g.groups < 0).all() else g.groups.shape[0]
+    s.gmax = g.gmax
+    for k, t in d.items():
+        setattr(s, k, t.data_ptr())
+    g.struct = s
+
+
+_STORE = {"f32": (L.STORE_F32, torch.float32), "f16": (L.STORE_F16, torch.float16), "i8": (L.STORE_I8, torch.int8)}
+
+
+@dataclass
+class ReplicaSpec:
+    graph: int
+    row0: int
+    nrung: int = 1
+    ladder: int = -1
+    rung: int = 0          # initial rung (row = row0 + rung)
+
+
+class Batch:
+    """Many replicas (of possibly different lattices) run by one engine call."""
+
+    def __init__(self, nc: int, graphs: List[GraphTables], reps: List[ReplicaSpec], beta: Sequence[float],
+                 h: Sequence[float], init_spins: List[np.ndarray], store: str = "f32", meas: int = 0,
+                 nbins: int = 32, nframes: int = 1, seed: int = 0x5EED2026, wolff: bool = False,
+                 recompute_every: int = 0, device="cuda", serial_base: int = 0):
+        self.nc = nc
+        self.device = torch.device(device)
+        if nc == 1:
+            store = "i8"
+        self.store_code, self.dtype = _STORE[store]
+        self.graphs = graphs
+        self.reps = reps
+        for g in graphs:
+            upload_graph(g, self.device)
+        nrep = len(reps)
+        nrow = len(beta)
+        self.nrow = nrow
+        # spin pool
+        offs, soffs, tot, stot = [], [], 0, 0
+        for r in reps:
+            n = graphs[r.graph].n
+            offs.append(tot)
+            soffs.append(stot)
+            tot += nc * n
+            stot += n
+        self.spin_off = offs
+        pool = np.zeros(max(tot, 1), np.float32)
+        for k, (r, sp) in enumerate(zip(reps, init_spins)):
+            n = graphs[r.graph].n
+            a = np.asarray(sp, np.float64).reshape(n, -1)
+            if nc == 1:
+                pool[offs[k]:offs[k] + n] = np.where(a[:, 0] >= 0, 1.0, -1.0)
+            else:
+                pool[offs[k]:offs[k] + nc * n] = a[:, :nc].T.reshape(-1)
+        self.spins = torch.from_numpy(pool).to(self.device).to(self.dtype)
+        f64 = dict(dtype=torch.float64, device=self.device)
+        i32 = dict(dtype=torch.int32, device=self.device)
+        self.row_beta = torch.tensor(np.asarray(beta, np.float32), device=self.device)
+        self.row_h = torch.tensor(np.asarray(h, np.float32), device=self.device)
+        rung = np.array([r.rung for r in reps], np.int32)
+        who = -np.ones(nrow, np.int32)
+        for k, r in enumerate(reps):
+            if not (0 <= r.rung < r.nrung) or r.row0 < 0 or r.row0 + r.nrung > nrow:
+                raise ValueError("replica %d: rung/rows outside the batch" % k)
+            if who[r.row0 + r.rung] >= 0:
+                raise ValueError("two replicas on row %d" % (r.row0 + r.rung))
+            who[r.row0 + r.rung] = k
+        for r in reps:
+            if r.ladder >= 0 and (who[r.row0:r.row0 + r.nrung] < 0).any():
+                raise ValueError("tempering ladder at row %d is not fully occupied" % r.row0)
+        self.rung = torch.from_numpy(rung).to(self.device)
+        self.who = torch.from_numpy(who).to(self.device)
+        self.state = torch.zeros(nrep * L.NSTATE, **f64)
+        self.acc = torch.zeros(nrow * L.NACC, **f64)
+        self.nbins = max(1, nbins)
+        self.bins = torch.zeros(nrow * self.nbins * L.NBINQ, **f64)
+        ng = max([g.groups.shape[0] if not (g.groups < 0).all() else 0 for g in graphs] + [0])
+        self.ngroup = ng
+        self.gacc = torch.zeros(max(1, nrow * (ng + 2) * (ng + 1)), **f64)
+        self.pt_acc = torch.zeros(nrow, **i32)
+        self.err = torch.zeros(1, **i32)
+        self.wstamp = torch.zeros(max(stot, 1), **i32) if wolff else None
+        self.wlist = torch.zeros(max(stot, 1), **i32) if wolff else None
+        # structs
+        self.h_graphs = (L.Graph * len(graphs))(*[g.struct for g in graphs])
+        self.h_reps = (L.Replica * nrep)()
+        for k, r in enumerate(reps):
+            x = self.h_reps[k]
+            x.graph, x.ladder, x.row0, x.nrung = r.graph, r.ladder, r.row0, r.nrung
+            x.serial = (serial_base + k) & 0x7FFFFFFF
+            x.spin_off, x.site_off = offs[k], soffs[k]
+        self.d_graphs = torch.frombuffer(bytearray(bytes(self.h_graphs)), dtype=torch.uint8).to(self.device)
+        self.d_reps = torch.frombuffer(bytearray(bytes(self.h_reps)), dtype=torch.uint8).to(self.device)
+        b = L.Batch()
+        b.nc, b.store, b.ngraph, b.nrep, b.nrow = nc, self.store_code, len(graphs), nrep, nrow
+        b.meas, b.nbins, b.nframes = meas, self.nbins, max(1, nframes)
+        b.seed_lo, b.seed_hi = seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF
+        b.recompute_every = recompute_every
+        b.graphs, b.reps, b.spins = self.d_graphs.data_ptr(), self.d_reps.data_ptr(), self.spins.data_ptr()
+        b.row_beta, b.row_h = self.row_beta.data_ptr(), self.row_h.data_ptr()
+        b.rung, b.who, b.state = self.rung.data_ptr(), self.who.data_ptr(), self.state.data_ptr()
+        b.acc, b.bins, b.gacc = self.acc.data_ptr(), self.bins.data_ptr(), self.gacc.data_ptr()
+        b.pt_acc, b.err = self.pt_acc.data_ptr(), self.err.data_ptr()
+        b.wstamp = self.wstamp.data_ptr() if wolff else None
+        b.wlist = self.wlist.data_ptr() if wolff else None
+        b.graphs_host = C.addressof(self.h_graphs)
+        b.reps_host = C.addressof(self.h_reps)
+        self.b = b
+        self.frames_per_row = max(1, nframes)
+
+    # ---------------------------------------------------------------------
+    @staticmethod
+    def _stream():
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def init(self) -> None:
+        L.check(L.lib().mcg_init(C.byref(self.b), self._stream()), "mcg_init")
+
+    def run(self, nthermal: int, nsweep: int, sweeps_per_frame: int = 1, algorithm: int = L.METROPOLIS,
+            pt_every: int = 0, kernel: int = L.KERNEL_AUTO, block: int = 0, reps_per_block: int = 0,
+            clusters_per_sweep: int = 0, sweep0: int = 0, frame0: int = 0) -> None:
+        p = L.Plan()
+        p.algorithm, p.nthermal, p.nsweep, p.sweeps_per_frame = algorithm, nthermal, nsweep, sweeps_per_frame
+        p.pt_every, p.kernel, p.block, p.reps_per_block = pt_every, kernel, block, reps_per_block
+        p.clusters_per_sweep, p.sweep0, p.frame0 = clusters_per_sweep, sweep0, frame0
+        L.check(L.lib().mcg_run(C.byref(self.b), C.byref(p), self._stream()), "mcg_run")
+
+    def sweeps(self, sweep0: int, nsweeps: int, algorithm: int = L.METROPOLIS, kernel: int = L.KERNEL_AUTO,
+               clusters_per_sweep: int = 0, block: int = 0) -> None:
+        p = L.Plan()
+        p.algorithm, p.kernel, p.clusters_per_sweep, p.sweeps_per_frame, p.block = \
+            algorithm, kernel, clusters_per_sweep, 1, block
+        L.check(L.lib().mcg_sweeps(C.byref(self.b), C.byref(p), sweep0, nsweeps, self._stream()), "mcg_sweeps")
+
+    def measure(self, frame: int) -> None:
+        L.check(L.lib().mcg_measure(C.byref(self.b), frame, self._stream()), "mcg_measure")
+
+    # ---------------------------------------------------------------------
+    def set_nframes(self, nframes: int) -> None:
+        self.b.nframes = max(1, nframes)
+
+    def accumulators(self) -> np.ndarray:
+        torch.cuda.synchronize()
+        return self.acc.cpu().numpy().reshape(self.nrow, L.NACC)
+
+    def bin_data(self) -> np.ndarray:
+        torch.cuda.synchronize()
+        return self.bins.cpu().numpy().reshape(self.nrow, self.nbins, L.NBINQ)
+
+    def group_acc(self) -> np.ndarray:
+        torch.cuda.synchronize()
+        ng = self.ngroup
+        return self.gacc.cpu().numpy()[: self.nrow * (ng + 2) * (ng + 1)].reshape(self.nrow, -1)
+
+    def state_data(self) -> np.ndarray:
+        torch.cuda.synchronize()
+        return self.state.cpu().numpy().reshape(-1, L.NSTATE)
+
+    def replica_spins(self, k: int) -> np.ndarray:
+        """[n, nc] spins of replica k (Ising: sigma*S is not applied)."""
+        n = self.graphs[self.reps[k].graph].n
+        o = self.spin_off[k]
+        a = self.spins[o:o + self.nc * n].float().cpu().numpy().reshape(self.nc, n).T
+        return a
+
+    def error_flag(self) -> int:
+        torch.cuda.synchronize()
+        return int(self.err.item())
