@@ -1,0 +1,146 @@
+"""Benchmark: spin updates/s of the batched MI355X engine on the XY
+Binder-scan workload (7 square lattices L = 16..128 mixed in one launch, each
+a 64-rung parallel-tempering ladder over T in [0.7, 1.2], swap and measure
+every sweep, FP32 spins, FP64 accumulators).
+
+One step = one sweep of every replica (2,166,784 spin updates) + one
+tempering swap pass + one measured frame.  Printed as one JSON line with:
+  value        whole-job spin updates/s (inputs resident in HBM)
+  roofline     dominant kernel (k_lds) vs the 8 TB/s HBM peak, from
+               HIP-event timing of every launch of it in the timed region and
+               2 x (state bytes) algorithmic traffic per site update
+  cpu_baseline the FP64 reference-algorithm restatement (oracle), random
+               sequential Metropolis with glibc rand(), on the host cores
+  plain        the same trajectory with one launch per replica per colour per
+               sweep (global-memory stencil) and the resulting speedup
+
+python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-plain]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "spin updates/s (replica·sites/s) + % HBM roofline; speedup vs plain at 2σ U4"
+HBM_PEAK = 8.0e12
+
+
+def _cpu_worker(args):
+    L, T, nsweep, seed = args
+    from mcsolver_amd import lattice as lat
+    from oracle import mcref
+    Lt = lat.square(L, J=-1.0)
+    a = lat.on_args(Lt, T, 0.0, 0, nsweep)
+    t0 = time.perf_counter()
+    mcref.on_run(2, a, seed=seed)
+    return Lt.n * nsweep, time.perf_counter() - t0
+
+
+def cpu_baseline(target_s: float = 10.0):
+    """Reference algorithm (oracle restatement of xyLib.c MCMainFunction,
+    random-sequential Metropolis + full observable set, FP64, glibc rand())
+    on L=32 points of the same T ladder, one process per point as
+    mcsolver/win.py:81-103 pumps them."""
+    P = min(16, len(os.sched_getaffinity(0)))
+    # calibrate one point on this core
+    upd, dt = _cpu_worker((32, 0.9, 200, 1))
+    rate1 = upd / dt
+    nsweep = max(200, int(target_s * rate1 / 1024))
+    temps = [0.7 + 0.5 * k / (P - 1 if P > 1 else 1) for k in range(P)]
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(P) as pool:
+        res = pool.map(_cpu_worker, [(32, T, nsweep, 100 + k) for k, T in enumerate(temps)])
+    wall = time.perf_counter() - t0
+    tot = sum(r[0] for r in res)
+    return {"value": tot / wall, "unit": "spin updates/s", "cores": P, "kind": "port",
+            "sample": "XY L=32, %d temperatures of the [0.7,1.2] ladder, %d sweeps each, FP64 oracle restatement of "
+                      "xyLib.c MCMainFunction (random-sequential Metropolis, glibc rand, full observables), one "
+                      "process per point; %.1f s wall" % (P, nsweep, wall),
+            "per_core": tot / sum(r[1] for r in res)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=20)
+    a = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print("bench.py runs on one GPU only: replicas shard across GPUs with no communication during sweeps, "
+              "so N>1 is the same kernel on 1/N of the batch; multi-GPU runs are not part of this build.",
+              file=sys.stderr)
+        sys.exit(2)
+    # CPU baseline first (forks before the GPU is initialised)
+    cpu = None if a.no_cpu else cpu_baseline()
+
+    import torch
+    from mcsolver_amd import _lib as L
+    from mcsolver_amd import workloads as W
+
+    torch.cuda.set_device(0)
+    K, Wm = a.steps, a.warmup
+    wl = W.build("c2", nframes=K + Wm, nbins=16)
+    b = wl.batch
+    b.init()
+    b.run(0, Wm, 1, pt_every=wl.pt_every, block=a.block)
+    torch.cuda.synchronize()
+    timing = (C.c_double * 4)()
+    st, en = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    st.record()
+    p = L.Plan()
+    p.algorithm, p.nthermal, p.nsweep, p.sweeps_per_frame, p.pt_every = L.METROPOLIS, 0, K, 1, wl.pt_every
+    p.kernel, p.block, p.sweep0, p.frame0 = L.KERNEL_AUTO, a.block, Wm, Wm
+    p.timing = C.cast(timing, C.POINTER(C.c_double))
+    L.check(L.lib().mcg_run(C.byref(b.b), C.byref(p), b._stream()), "mcg_run")
+    en.record()
+    torch.cuda.synchronize()
+    ms = st.elapsed_time(en)
+    ups = wl.updates_per_sweep
+    value = ups * K / (ms / 1e3)
+    bpu = W.bytes_per_update(2, "f32")
+    k_ms, k_launch, k_sweeps, k_upd = timing[0], timing[1], timing[2], timing[3]
+    achieved = (k_upd * bpu) / (k_ms / 1e3) if k_ms > 0 else 0.0
+    roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "k_lds<2,float>",
+            "avg_launch_us": 1e3 * k_ms / max(k_launch, 1), "launches": int(k_launch),
+            "algorithmic_bytes_per_launch": bpu * k_upd / max(k_launch, 1)}
+    acc = b.accumulators()
+    assert b.error_flag() == 0
+    assert (acc[:, L.A_COUNT] == K + Wm).all(), "every temperature row must have one frame per sweep"
+    out = {"metric": METRIC, "value": value, "unit": "spin updates/s", "n_gpus": 1, "steps": K, "warmup": Wm,
+           "ms_per_step": ms / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "configs[2] XY batched Binder scan", "lattices": list(W.C2_SIZES),
+                      "rungs_per_ladder": 64, "replicas": len(b.reps), "spins_per_sweep": ups, "pt_every": 1,
+                      "measure_every": 1, "spin_storage": "fp32", "accumulators": "fp64"},
+           "roofline": roof, "cpu_baseline": cpu}
+    if not a.no_plain:
+        wp = W.build("c2", nframes=a.plain_steps, nbins=1)
+        wp.batch.init()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        wp.batch.run(0, a.plain_steps, 1, pt_every=wp.pt_every, kernel=L.KERNEL_PLAIN)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        pv = wp.updates_per_sweep * a.plain_steps / dt
+        out["plain"] = {"value": pv, "unit": "spin updates/s", "steps": a.plain_steps,
+                        "kernel": "k_pass, one launch per replica per colour per sweep + per-replica measure"}
+        out["speedup_vs_plain"] = value / pv
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

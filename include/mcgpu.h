@@ -66,7 +66,7 @@ enum { MCG_NBINQ = 5 };
 typedef struct mcg_graph {
   int32_t n, zmax, ncol, ncoup, ncls;
   int32_t nlat, ncirc, nr, nclu, ngroup, gmax;
-  int32_t pad_;
+  int32_t npat;            /* link patterns (0 = none: LDS kernel reads link[]) */
   const uint32_t *link;    /* [n*zmax] */
   const float *coup;       /* [ncoup*9] row-major 3x3 (E = s_i^T J s_j), unscaled */
   const uint8_t *cls;      /* [n] site class */
@@ -80,6 +80,13 @@ typedef struct mcg_graph {
   const uint32_t *rlink;   /* [nr*zmax] block-spin links, packed like link */
   const int32_t *groups;   /* [ngroup*gmax], -1 padded */
   const int32_t *rrank;    /* [n] index into rorb, or -1 */
+  /* colour-permuted view used by the LDS-resident kernel: position p holds
+     site col_site[p]; pos[site] = p.  pat[p] selects a row of pattab:
+     zmax words (signed position delta in the low 24 bits | coupling << 24)
+     followed by the site class.  Periodic lattices have few patterns. */
+  const int32_t *pos;      /* [n] */
+  const uint8_t *pat;      /* [n] */
+  const int32_t *pattab;   /* [npat*(zmax+1)] */
 } mcg_graph;
 /* Ising couplings: coup[9c+0] = J*S_target (folded, used by the sweep and the
  * energy), coup[9c+1] = J (raw, used by the block-spin energy). */
@@ -142,6 +149,10 @@ typedef struct mcg_plan {
   int32_t clusters_per_sweep; /* Wolff: clusters per replica per sweep (0 = n) */
   int32_t frame0;          /* index of the first measured frame of this call */
   int64_t sweep0;          /* global sweep index of the first sweep (Philox counter) */
+  double *timing;          /* host double[4] or NULL: when set, mcg_run brackets every
+                              launch of the sweep kernel with HIP events on the stream
+                              and returns [0] total ms, [1] launches, [2] sweeps,
+                              [3] site updates of those launches */
 } mcg_plan;
 
 /*

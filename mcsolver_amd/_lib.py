@@ -29,9 +29,9 @@ P = C.c_void_p
 
 class Graph(C.Structure):
     _fields_ = [(k, C.c_int32) for k in ("n", "zmax", "ncol", "ncoup", "ncls", "nlat", "ncirc", "nr", "nclu",
-                                         "ngroup", "gmax", "pad_")] + \
+                                         "ngroup", "gmax", "npat")] + \
                [(k, P) for k in ("link", "coup", "cls", "clsdat", "col_ptr", "col_site", "pairs", "circ", "rorb",
-                                 "rclu", "rlink", "groups", "rrank")]
+                                 "rclu", "rlink", "groups", "rrank", "pos", "pat", "pattab")]
 
 
 class Replica(C.Structure):
@@ -50,12 +50,12 @@ class Batch(C.Structure):
 class Plan(C.Structure):
     _fields_ = [(k, C.c_int32) for k in ("algorithm", "nthermal", "nsweep", "sweeps_per_frame", "pt_every", "kernel",
                                          "block", "reps_per_block", "clusters_per_sweep", "frame0")] + \
-               [("sweep0", C.c_int64)]
+               [("sweep0", C.c_int64), ("timing", C.POINTER(C.c_double))]
 
 
-assert C.sizeof(Graph) == 48 + 13 * 8
+assert C.sizeof(Graph) == 48 + 16 * 8
 assert C.sizeof(Replica) == 40
-assert C.sizeof(Plan) == 48
+assert C.sizeof(Plan) == 56
 
 _lib = None
 

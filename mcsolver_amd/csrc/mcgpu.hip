@@ -87,6 +87,30 @@ struct Spins {
   __device__ __forceinline__ void set(int i, int c, float v) const { SIO<ST>::st(p + (size_t)c * n + i, v); }
 };
 
+// colour-permuted replica (LDS): site i lives at position pos[i]
+template <typename ST>
+struct PSpins {
+  ST* p;
+  int n;
+  const int32_t* pos;
+  __device__ __forceinline__ float get(int i, int c) const { return SIO<ST>::ld(p + (size_t)c * n + pos[i]); }
+  __device__ __forceinline__ void set(int i, int c, float v) const { SIO<ST>::st(p + (size_t)c * n + pos[i], v); }
+};
+
+// neighbour accessors: the global link table (site order) or a pattern row
+// (colour-permuted positions, LDS)
+struct GlobalLinks {
+  const uint32_t* lk;
+  __device__ __forceinline__ int nbr(int l) const { return (int)(lk[l] & 0xFFFFFFu); }
+  __device__ __forceinline__ uint32_t cid(int l) const { return lk[l] >> 24; }
+};
+struct PatLinks {
+  const int32_t* row;
+  int p;
+  __device__ __forceinline__ int nbr(int l) const { return p + ((int)((uint32_t)row[l] << 8) >> 8); }
+  __device__ __forceinline__ uint32_t cid(int l) const { return (uint32_t)row[l] >> 24; }
+};
+
 static constexpr double REF_PI = 3.1415926535;  // heisenbergLib.c:6
 
 // ---------------------------------------------------------------------------
@@ -98,27 +122,22 @@ static constexpr double REF_PI = 3.1415926535;  // heisenbergLib.c:6
 // (heisenbergLib.c:441-473, xyLib.c:382-409); Ising: flip (isingLib.c:238-254).
 // Accept if beta*dE <= 0 or u < exp(-beta*dE).  Accepted O(n) spins are
 // renormalised to |S|.
-template <int NC, typename ST>
-__device__ __forceinline__ void metro_site(const Spins<ST>& sp, int i, const uint32_t* __restrict__ link, int zmax,
-                                           const float* coup, const uint8_t* __restrict__ cls, const float* clsdat,
-                                           float beta, float h, uint64_t sweep, uint32_t key0, uint32_t key1,
-                                           double& dEacc, double* dMacc) {
-  const int ci = cls[i];
+// Random words: w_a = r_z (Heisenberg), w_b = phi (XY, Heisenberg),
+// w_c = acceptance.
+template <int NC, typename SV, typename LK>
+__device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, int zmax, const float* coup, int ci,
+                                           const float* clsdat, float beta, float h, uint32_t w_a, uint32_t w_b,
+                                           uint32_t w_c, double& dEacc, double* dMacc) {
   const float S = clsdat[ci * 4 + 0];
-  const uint32_t* lk = link + (size_t)i * zmax;
-  const u32x4 x = philox4x32_10(u32x4{(uint32_t)i, (uint32_t)sweep, (uint32_t)(sweep >> 32), TAG_METRO}, key0, key1);
   if constexpr (NC == 1) {
     float f = 0.0f;
-    for (int l = 0; l < zmax; ++l) {
-      const uint32_t w = lk[l];
-      f = fmaf(coup[(w >> 24) * 9], sp.get((int)(w & 0xFFFFFFu), 0), f);
-    }
+    for (int l = 0; l < zmax; ++l) f = fmaf(coup[lk.cid(l) * 9], sp.get(lk.nbr(l), 0), f);
     const float sg = sp.get(i, 0);
     const float s = sg * S;
     const float t = f - h;
     const float dE = (-2.0f * s) * t;
     const float db = dE * beta;
-    if (db <= 0.0f || u24(x.z) < exp_neg(-db)) {
+    if (db <= 0.0f || u24(w_c) < exp_neg(-db)) {
       sp.set(i, 0, -sg);
       dEacc += (double)dE;
       dMacc[0] += (double)(-2.0f * s);
@@ -129,9 +148,8 @@ __device__ __forceinline__ void metro_site(const Spins<ST>& sp, int i, const uin
     const float s2 = NC == 3 ? sp.get(i, 2) : 0.0f;
     float H0 = 0.0f, H1 = 0.0f, H2 = 0.0f;
     for (int l = 0; l < zmax; ++l) {
-      const uint32_t w = lk[l];
-      const int j = (int)(w & 0xFFFFFFu);
-      const float* J = coup + (w >> 24) * 9;
+      const int j = lk.nbr(l);
+      const float* J = coup + lk.cid(l) * 9;
       const float t0 = sp.get(j, 0), t1 = sp.get(j, 1);
       if constexpr (NC == 3) {
         const float t2 = sp.get(j, 2);
@@ -145,14 +163,14 @@ __device__ __forceinline__ void metro_site(const Spins<ST>& sp, int i, const uin
     }
     float rx, ry, rz = 0.0f;
     if constexpr (NC == 3) {
-      rz = fmaf(u24(x.x), 2.0f, -1.0f);
+      rz = fmaf(u24(w_a), 2.0f, -1.0f);
       const float rho = sqrtf(fmaxf(fmaf(-rz, rz, 1.0f), 0.0f));
       float c, sn;
-      sincos_turn(x.y, &c, &sn);
+      sincos_turn(w_b, &c, &sn);
       rx = rho * c;
       ry = rho * sn;
     } else {
-      sincos_turn(x.y, &rx, &ry);
+      sincos_turn(w_b, &rx, &ry);
     }
     float p = s0 * rx;
     p = fmaf(s1, ry, p);
@@ -175,7 +193,7 @@ __device__ __forceinline__ void metro_site(const Spins<ST>& sp, int i, const uin
     }
     dE = fmaf(-h, NC == 3 ? d2 : d0, dE);
     const float db = dE * beta;
-    if (db <= 0.0f || u24(x.z) < exp_neg(-db)) {
+    if (db <= 0.0f || u24(w_c) < exp_neg(-db)) {
       float nn = n0 * n0;
       nn = fmaf(n1, n1, nn);
       if constexpr (NC == 3) nn = fmaf(n2, n2, nn);
@@ -194,9 +212,69 @@ __device__ __forceinline__ void metro_site(const Spins<ST>& sp, int i, const uin
   }
 }
 
-// spin vector of site i (Ising: sigma*S)
+// One Philox draw (128 bits) serves G sites of a colour class: 4 Ising
+// updates (one acceptance word each), 2 XY updates (angle + acceptance), or
+// 1 Heisenberg update (r_z, angle, acceptance).  Group q of colour c covers
+// colour-list positions q*G .. q*G+G-1; counter (q, sweep, c<<8 | TAG).
+template <int NC>
+struct SitesPerDraw {
+  static constexpr int G = NC == 1 ? 4 : NC == 2 ? 2 : 1;
+};
+
 template <int NC, typename ST>
-__device__ __forceinline__ void spin_vec(const Spins<ST>& sp, int i, const uint8_t* __restrict__ cls, const float* clsdat,
+__device__ __forceinline__ void metro_group(const Spins<ST>& sp, const int32_t* __restrict__ cs, int cnt, int q,
+                                            int colour, const uint32_t* __restrict__ link, int zmax,
+                                            const float* coup, const uint8_t* __restrict__ cls, const float* clsdat,
+                                            float beta, float h, uint64_t sweep, uint32_t key0, uint32_t key1,
+                                            double& dEacc, double* dMacc) {
+  constexpr int G = SitesPerDraw<NC>::G;
+  const u32x4 x = philox4x32_10(
+      u32x4{(uint32_t)q, (uint32_t)sweep, (uint32_t)(sweep >> 32), TAG_METRO | ((uint32_t)colour << 8)}, key0, key1);
+  const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int p = q * G + g;
+    if (p < cnt) {
+      const int i = cs[p];
+      const GlobalLinks lk{link + (size_t)i * zmax};
+      const uint32_t wa = NC == 3 ? w[0] : 0u;
+      const uint32_t wb = NC == 3 ? w[1] : NC == 2 ? w[2 * g] : 0u;
+      const uint32_t wc = NC == 3 ? w[2] : NC == 2 ? w[2 * g + 1] : w[g];
+      metro_core<NC>(sp, i, lk, zmax, coup, (int)cls[i], clsdat, beta, h, wa, wb, wc, dEacc, dMacc);
+    }
+  }
+}
+
+// same, on the colour-permuted LDS replica: colour c occupies positions
+// b .. b+cnt-1, the neighbours come from the pattern row of each position
+template <int NC, typename ST>
+__device__ __forceinline__ void metro_group_pat(const Spins<ST>& sp, int b, int cnt, int q, int colour,
+                                                const uint8_t* pat, const int32_t* pattab, int zmax,
+                                                const float* coup, const float* clsdat, float beta, float h,
+                                                uint64_t sweep, uint32_t key0, uint32_t key1, double& dEacc,
+                                                double* dMacc) {
+  constexpr int G = SitesPerDraw<NC>::G;
+  const u32x4 x = philox4x32_10(
+      u32x4{(uint32_t)q, (uint32_t)sweep, (uint32_t)(sweep >> 32), TAG_METRO | ((uint32_t)colour << 8)}, key0, key1);
+  const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int lp = q * G + g;
+    if (lp < cnt) {
+      const int p = b + lp;
+      const int32_t* row = pattab + (int)pat[p] * (zmax + 1);
+      const PatLinks lk{row, p};
+      const uint32_t wa = NC == 3 ? w[0] : 0u;
+      const uint32_t wb = NC == 3 ? w[1] : NC == 2 ? w[2 * g] : 0u;
+      const uint32_t wc = NC == 3 ? w[2] : NC == 2 ? w[2 * g + 1] : w[g];
+      metro_core<NC>(sp, p, lk, zmax, coup, row[zmax], clsdat, beta, h, wa, wb, wc, dEacc, dMacc);
+    }
+  }
+}
+
+// spin vector of site i (Ising: sigma*S)
+template <int NC, typename SV>
+__device__ __forceinline__ void spin_vec(const SV& sp, int i, const uint8_t* __restrict__ cls, const float* clsdat,
                                          float* v) {
   if constexpr (NC == 1) {
     v[0] = sp.get(i, 0) * clsdat[cls[i] * 4];
@@ -209,8 +287,8 @@ __device__ __forceinline__ void spin_vec(const Spins<ST>& sp, int i, const uint8
 }
 
 // per-site energy e_i (FP32): 1/2 s_i.H_i + onsite - h s_axis
-template <int NC, typename ST>
-__device__ __forceinline__ float site_energy(const Spins<ST>& sp, int i, const uint32_t* __restrict__ link, int zmax,
+template <int NC, typename SV>
+__device__ __forceinline__ float site_energy(const SV& sp, int i, const uint32_t* __restrict__ link, int zmax,
                                              const float* coup, const uint8_t* __restrict__ cls, const float* clsdat,
                                              float h) {
   const int ci = cls[i];
@@ -288,14 +366,14 @@ struct FrameCtx {
 };
 
 // majority spin of block-spin site (heisenbergLib.c:255-267, isingLib.c:133-150)
-template <int NC, typename ST>
-__device__ __forceinline__ void majority(const Spins<ST>& sp, const mcg_graph& G, const float* clsdat, int site,
+template <int NC, typename SV>
+__device__ __forceinline__ void majority(const SV& sp, const mcg_graph& G, const float* clsdat, int site,
                                          uint64_t frame, uint32_t key0, uint32_t key1, float* v) {
   const int ri = G.rrank[site];
   float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
   for (int k = 0; k < G.nclu; ++k) {
     float t[3];
-    spin_vec<NC, ST>(sp, G.rclu[ri * G.nclu + k], G.cls, clsdat, t);
+    spin_vec<NC>(sp, G.rclu[ri * G.nclu + k], G.cls, clsdat, t);
     a0 += t[0];
     a1 += t[1];
     a2 += t[2];
@@ -343,8 +421,8 @@ __device__ __forceinline__ float signed_area(const float* a, const float* b, con
 // Measures one frame.  Tracked E and M (FP64) of the replica are passed in
 // st[]; everything else is computed from the spins.  Only the group leader
 // (lt == 0) writes the accumulators.
-template <int NC, typename ST>
-__device__ void measure_frame(const Spins<ST>& sp, const FrameCtx& C, const mcg_batch& B, const double* st, int row,
+template <int NC, typename SV>
+__device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& B, const double* st, int row,
                               int64_t frame, uint32_t key0, uint32_t key1, float h) {
   const mcg_graph& G = *C.G;
   const int lt = C.lt, TPR = C.TPR;
@@ -356,15 +434,15 @@ __device__ void measure_frame(const Spins<ST>& sp, const FrameCtx& C, const mcg_
     for (int k = 0; k < RK; ++k) v[k] = 0.0;
     for (int p = lt; p < G.nlat; p += TPR) {
       float a[3], b[3];
-      spin_vec<NC, ST>(sp, G.pairs[2 * p], G.cls, C.clsdat, a);
-      spin_vec<NC, ST>(sp, G.pairs[2 * p + 1], G.cls, C.clsdat, b);
+      spin_vec<NC>(sp, G.pairs[2 * p], G.cls, C.clsdat, a);
+      spin_vec<NC>(sp, G.pairs[2 * p + 1], G.cls, C.clsdat, b);
       v[0] += a[0]; v[1] += a[1]; v[2] += a[2];
       v[3] += b[0]; v[4] += b[1]; v[5] += b[2];
       v[6] += (double)(a[0] * b[0] + a[1] * b[1] + a[2] * b[2]);
     }
     if (lt == 0) {
       float t[3];
-      spin_vec<NC, ST>(sp, G.n - 1, G.cls, C.clsdat, t);
+      spin_vec<NC>(sp, G.n - 1, G.cls, C.clsdat, t);
       v[7] = t[0];
     }
     group_sum8(v, C.s_red, C.g, TPR);
@@ -377,9 +455,9 @@ __device__ void measure_frame(const Spins<ST>& sp, const FrameCtx& C, const mcg_
     for (int p = lt; p < G.ncirc; p += TPR) {
       const int ia = G.circ[3 * p], ib = G.circ[3 * p + 1], ic = G.circ[3 * p + 2];
       float a[3], b[3], c[3];
-      spin_vec<NC, ST>(sp, ia, G.cls, C.clsdat, a);
-      spin_vec<NC, ST>(sp, ib, G.cls, C.clsdat, b);
-      spin_vec<NC, ST>(sp, ic, G.cls, C.clsdat, c);
+      spin_vec<NC>(sp, ia, G.cls, C.clsdat, a);
+      spin_vec<NC>(sp, ib, G.cls, C.clsdat, b);
+      spin_vec<NC>(sp, ic, G.cls, C.clsdat, c);
       v[0] += (double)signed_area(a, b, c, fabsf(C.clsdat[G.cls[ia] * 4]), fabsf(C.clsdat[G.cls[ib] * 4]),
                                   fabsf(C.clsdat[G.cls[ic] * 4]));
     }
@@ -396,12 +474,12 @@ __device__ void measure_frame(const Spins<ST>& sp, const FrameCtx& C, const mcg_
       const bool ca = G.rrank[ia] >= 0, cb = G.rrank[ib] >= 0;
       float ma[3] = {0, 0, 0}, mb[3] = {0, 0, 0};
       if (ca) {
-        majority<NC, ST>(sp, G, C.clsdat, ia, frame, key0, key1, ma);
+        majority<NC>(sp, G, C.clsdat, ia, frame, key0, key1, ma);
         v[0] += ma[0]; v[1] += ma[1]; v[2] += ma[2];
         w[0] += 1.0;
       }
       if (cb) {
-        majority<NC, ST>(sp, G, C.clsdat, ib, frame, key0, key1, mb);
+        majority<NC>(sp, G, C.clsdat, ib, frame, key0, key1, mb);
         v[3] += mb[0]; v[4] += mb[1]; v[5] += mb[2];
         w[1] += 1.0;
       }
@@ -413,14 +491,14 @@ __device__ void measure_frame(const Spins<ST>& sp, const FrameCtx& C, const mcg_
     for (int ri = lt; ri < G.nr; ri += TPR) {
       const int j = G.rorb[ri];
       float m[3];
-      majority<NC, ST>(sp, G, C.clsdat, j, frame, key0, key1, m);
+      majority<NC>(sp, G, C.clsdat, j, frame, key0, key1, m);
       float cc = 0.0f;
       for (int l = 0; l < G.zmax; ++l) {
         const uint32_t wd = G.rlink[(size_t)ri * G.zmax + l];
         const int t = (int)(wd & 0xFFFFFFu);
         const float* J = C.coup + (wd >> 24) * 9;
         float mt[3];
-        majority<NC, ST>(sp, G, C.clsdat, t, frame, key0, key1, mt);
+        majority<NC>(sp, G, C.clsdat, t, frame, key0, key1, mt);
         if constexpr (NC == 1) {
           cc += J[1] * m[0] * mt[0];
         } else {
@@ -434,7 +512,7 @@ __device__ void measure_frame(const Spins<ST>& sp, const FrameCtx& C, const mcg_
       // on-site part with the site's actual spin (heisenbergLib.c:798-800,
       // isingLib.c:390)
       float s[3];
-      spin_vec<NC, ST>(sp, j, G.cls, C.clsdat, s);
+      spin_vec<NC>(sp, j, G.cls, C.clsdat, s);
       const int ci = G.cls[j];
       float eo;
       if constexpr (NC == 1)
@@ -460,7 +538,7 @@ __device__ void measure_frame(const Spins<ST>& sp, const FrameCtx& C, const mcg_
         const int o = G.groups[(g0 + q) * G.gmax + k];
         if (o < 0) continue;
         float s[3];
-        spin_vec<NC, ST>(sp, o, G.cls, C.clsdat, s);
+        spin_vec<NC>(sp, o, G.cls, C.clsdat, s);
         v[3 * q] += s[0]; v[3 * q + 1] += s[1]; v[3 * q + 2] += s[2];
       }
     }
@@ -566,15 +644,15 @@ __device__ void measure_frame(const Spins<ST>& sp, const FrameCtx& C, const mcg_
 }
 
 // full FP64 recomputation of E and M of one replica (group-parallel)
-template <int NC, typename ST>
-__device__ void recompute(const Spins<ST>& sp, const mcg_graph& G, const float* coup, const float* clsdat, float h,
+template <int NC, typename SV>
+__device__ void recompute(const SV& sp, const mcg_graph& G, const float* coup, const float* clsdat, float h,
                           double* s_red, int g, int lt, int TPR, double* out4) {
   double v[RK];
   for (int k = 0; k < RK; ++k) v[k] = 0.0;
   for (int i = lt; i < G.n; i += TPR) {
-    v[0] += (double)site_energy<NC, ST>(sp, i, G.link, G.zmax, coup, G.cls, clsdat, h);
+    v[0] += (double)site_energy<NC>(sp, i, G.link, G.zmax, coup, G.cls, clsdat, h);
     float s[3];
-    spin_vec<NC, ST>(sp, i, G.cls, clsdat, s);
+    spin_vec<NC>(sp, i, G.cls, clsdat, s);
     v[1] += s[0];
     v[2] += s[1];
     v[3] += s[2];
@@ -604,9 +682,49 @@ static constexpr int FRM = 96;  // frame scratch doubles per group
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-__host__ __device__ inline size_t lds_fixed_bytes(int ncoup, int ncls, int nwaves, int R) {
-  return align16((size_t)(ncoup * 9 + ncls * 4) * 4) + align16((size_t)nwaves * RK * 8) +
-         align16((size_t)R * (MCG_NSTATE + FRM) * 8);
+__host__ __device__ inline size_t lds_fixed_bytes(int ncoup, int ncls, int npat, int zmax, int n, int nwaves,
+                                                  int R) {
+  size_t b = align16((size_t)(ncoup * 9 + ncls * 4) * 4);
+  if (npat > 0) b += align16((size_t)npat * (zmax + 1) * 4) + align16((size_t)n);
+  return b + align16((size_t)nwaves * RK * 8) + align16((size_t)R * (MCG_NSTATE + FRM) * 8);
+}
+
+__host__ __device__ inline size_t lds_replica_bytes(int nc, int n, int ssize) {
+  return align16((size_t)nc * n * ssize);
+}
+
+template <int NC, typename ST, typename SV>
+__device__ __forceinline__ void lds_sweep_tail(const SV& sv, const mcg_graph& G, const mcg_batch& B, const float* s_coup,
+                                               const float* s_cls, double* s_red, double* st, FrameCtx& C, int g,
+                                               int lt, int TPR, int row, float h, int64_t k, const SweepArgs& a,
+                                               bool meas, bool therm_end, bool rec, double& dE, double* dM,
+                                               uint32_t key0, uint32_t key1) {
+  double v[RK] = {dE, dM[0], dM[1], dM[2], 0, 0, 0, 0};
+  group_sum8(v, s_red, g, TPR);
+  if (lt == 0) {
+    st[MCG_S_E] += v[0];
+    st[MCG_S_M] += v[1];
+    st[MCG_S_M + 1] += v[2];
+    st[MCG_S_M + 2] += v[3];
+  }
+  dE = 0.0;
+  dM[0] = dM[1] = dM[2] = 0.0;
+  if (rec) {
+    double r4[4];
+    recompute<NC>(sv, G, s_coup, s_cls, h, s_red, g, lt, TPR, r4);
+    if (lt == 0) {
+      const double d = fabs(st[MCG_S_E] - r4[0]);
+      if (d > st[MCG_S_DRIFT]) st[MCG_S_DRIFT] = d;
+      st[MCG_S_E] = r4[0];
+      st[MCG_S_M] = r4[1];
+      st[MCG_S_M + 1] = r4[2];
+      st[MCG_S_M + 2] = r4[3];
+    }
+  }
+  if (therm_end && lt == 0) st[MCG_S_EREF] = st[MCG_S_E];
+  __syncthreads();
+  if (meas) measure_frame<NC>(sv, C, B, st, row, (k - a.nthermal + 1) / a.spf - 1 + a.frame0, key0, key1, h);
+  __syncthreads();
 }
 
 template <int NC, typename ST>
@@ -619,11 +737,18 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   const int g = threadIdx.x / TPR;
   const int lt = threadIdx.x - g * TPR;
   const int nwaves = blockDim.x / 64;
+  const bool perm = G.npat > 0;
+  const int n = G.n, zmax = G.zmax;
 
   extern __shared__ __align__(16) unsigned char smem[];
+  size_t off = 0;
   float* s_coup = (float*)smem;
   float* s_cls = s_coup + G.ncoup * 9;
-  size_t off = align16((size_t)(G.ncoup * 9 + G.ncls * 4) * 4);
+  off += align16((size_t)(G.ncoup * 9 + G.ncls * 4) * 4);
+  int32_t* s_pat = (int32_t*)(smem + off);
+  if (perm) off += align16((size_t)G.npat * (zmax + 1) * 4);
+  uint8_t* s_pid = (uint8_t*)(smem + off);
+  if (perm) off += align16((size_t)n);
   double* s_red = (double*)(smem + off);
   off += align16((size_t)nwaves * RK * 8);
   double* s_st = (double*)(smem + off);  // R * NSTATE
@@ -633,12 +758,21 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
 
   for (int t = threadIdx.x; t < G.ncoup * 9; t += blockDim.x) s_coup[t] = G.coup[t];
   for (int t = threadIdx.x; t < G.ncls * 4; t += blockDim.x) s_cls[t] = G.clsdat[t];
+  if (perm) {
+    for (int t = threadIdx.x; t < G.npat * (zmax + 1); t += blockDim.x) s_pat[t] = G.pattab[t];
+    for (int t = threadIdx.x; t < n; t += blockDim.x) s_pid[t] = G.pat[t];
+  }
 
   const int rep = item_reps[w.off + g];
   const mcg_replica rp = B.reps[rep];
   ST* gsp = (ST*)B.spins + rp.spin_off;
-  const Spins<ST> sp{s_spin + (size_t)g * NC * G.n, G.n};
-  for (int t = lt; t < NC * G.n; t += TPR) sp.p[t] = gsp[t];
+  ST* my = s_spin + lds_replica_bytes(NC, n, sizeof(ST)) / sizeof(ST) * g;
+  if (perm) {
+    for (int c = 0; c < NC; ++c)
+      for (int t = lt; t < n; t += TPR) my[(size_t)c * n + t] = gsp[(size_t)c * n + G.col_site[t]];
+  } else {
+    for (int t = lt; t < NC * n; t += TPR) my[t] = gsp[t];
+  }
   double* st = s_st + g * MCG_NSTATE;
   if (lt < MCG_NSTATE) st[lt] = B.state[(size_t)rep * MCG_NSTATE + lt];
   __syncthreads();
@@ -648,49 +782,45 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   const float beta = B.row_beta[row], h = B.row_h[row];
   double dE = 0.0, dM[3] = {0.0, 0.0, 0.0};
   FrameCtx C{&G, s_coup, s_cls, s_red, s_frm + g * FRM, g, lt, TPR};
+  const Spins<ST> sp{my, n};
+  const PSpins<ST> psp{my, n, G.pos};
+  constexpr int GD = SitesPerDraw<NC>::G;
 
   for (int s = 0; s < a.nsweeps; ++s) {
     const int64_t k = a.k_begin + s;
     const uint64_t kk = (uint64_t)(a.philox_base + k);
     for (int c = 0; c < G.ncol; ++c) {
-      const int b = G.col_ptr[c], e = G.col_ptr[c + 1];
-      for (int j = b + lt; j < e; j += TPR)
-        metro_site<NC, ST>(sp, G.col_site[j], G.link, G.zmax, s_coup, G.cls, s_cls, beta, h, kk, key0, key1, dE, dM);
+      const int b = G.col_ptr[c], cnt = G.col_ptr[c + 1] - b;
+      const int ngrp = (cnt + GD - 1) / GD;
+      if (perm) {
+        for (int q = lt; q < ngrp; q += TPR)
+          metro_group_pat<NC, ST>(sp, b, cnt, q, c, s_pid, s_pat, zmax, s_coup, s_cls, beta, h, kk, key0, key1, dE,
+                                  dM);
+      } else {
+        for (int q = lt; q < ngrp; q += TPR)
+          metro_group<NC, ST>(sp, G.col_site + b, cnt, q, c, G.link, zmax, s_coup, G.cls, s_cls, beta, h, kk, key0,
+                              key1, dE, dM);
+      }
       __syncthreads();
     }
     const bool meas = k >= a.nthermal && ((k - a.nthermal + 1) % a.spf) == 0;
     const bool therm_end = (k + 1 == a.nthermal);
     const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
     if (meas || therm_end || rec || s == a.nsweeps - 1) {
-      double v[RK] = {dE, dM[0], dM[1], dM[2], 0, 0, 0, 0};
-      group_sum8(v, s_red, g, TPR);
-      if (lt == 0) {
-        st[MCG_S_E] += v[0];
-        st[MCG_S_M] += v[1];
-        st[MCG_S_M + 1] += v[2];
-        st[MCG_S_M + 2] += v[3];
-      }
-      dE = 0.0;
-      dM[0] = dM[1] = dM[2] = 0.0;
-      if (rec) {
-        double r4[4];
-        recompute<NC, ST>(sp, G, s_coup, s_cls, h, s_red, g, lt, TPR, r4);
-        if (lt == 0) {
-          const double d = fabs(st[MCG_S_E] - r4[0]);
-          if (d > st[MCG_S_DRIFT]) st[MCG_S_DRIFT] = d;
-          st[MCG_S_E] = r4[0];
-          st[MCG_S_M] = r4[1];
-          st[MCG_S_M + 1] = r4[2];
-          st[MCG_S_M + 2] = r4[3];
-        }
-      }
-      if (therm_end && lt == 0) st[MCG_S_EREF] = st[MCG_S_E];
-      __syncthreads();
-      if (meas) measure_frame<NC, ST>(sp, C, B, st, row, (k - a.nthermal + 1) / a.spf - 1 + a.frame0, key0, key1, h);
-      __syncthreads();
+      if (perm)
+        lds_sweep_tail<NC, ST>(psp, G, B, s_coup, s_cls, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end, rec,
+                               dE, dM, key0, key1);
+      else
+        lds_sweep_tail<NC, ST>(sp, G, B, s_coup, s_cls, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end, rec,
+                               dE, dM, key0, key1);
     }
   }
-  for (int t = lt; t < NC * G.n; t += TPR) gsp[t] = sp.p[t];
+  if (perm) {
+    for (int c = 0; c < NC; ++c)
+      for (int t = lt; t < n; t += TPR) gsp[(size_t)c * n + G.col_site[t]] = my[(size_t)c * n + t];
+  } else {
+    for (int t = lt; t < NC * n; t += TPR) gsp[t] = my[t];
+  }
   if (lt < MCG_NSTATE) B.state[(size_t)rep * MCG_NSTATE + lt] = st[lt];
 }
 
@@ -705,13 +835,14 @@ __global__ __launch_bounds__(256) void k_pass(mcg_batch B, const int32_t* __rest
   const mcg_graph G = B.graphs[rp.graph];
   double dE = 0.0, dM[3] = {0.0, 0.0, 0.0};
   if (colour < G.ncol) {
-    const int b = G.col_ptr[colour], e = G.col_ptr[colour + 1];
-    const int j = b + blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < e) {
+    const int b = G.col_ptr[colour], cnt = G.col_ptr[colour + 1] - b;
+    const int ngrp = (cnt + SitesPerDraw<NC>::G - 1) / SitesPerDraw<NC>::G;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < ngrp) {
       const int row = rp.row0 + B.rung[rep];
       const Spins<ST> sp{(ST*)B.spins + rp.spin_off, G.n};
-      metro_site<NC, ST>(sp, G.col_site[j], G.link, G.zmax, G.coup, G.cls, G.clsdat, B.row_beta[row], B.row_h[row],
-                         (uint64_t)sweep, B.seed_lo, B.seed_hi ^ rp.serial, dE, dM);
+      metro_group<NC, ST>(sp, G.col_site + b, cnt, q, colour, G.link, G.zmax, G.coup, G.cls, G.clsdat,
+                          B.row_beta[row], B.row_h[row], (uint64_t)sweep, B.seed_lo, B.seed_hi ^ rp.serial, dE, dM);
     }
   }
   // block reduction, one FP64 atomic per value per block
@@ -759,7 +890,7 @@ __global__ __launch_bounds__(256) void k_measure(mcg_batch B, const int32_t* __r
   __syncthreads();
   if (mode & 1) {
     double r4[4];
-    recompute<NC, ST>(sp, G, G.coup, G.clsdat, h, s_red, 0, threadIdx.x, blockDim.x, r4);
+    recompute<NC>(sp, G, G.coup, G.clsdat, h, s_red, 0, threadIdx.x, blockDim.x, r4);
     if (threadIdx.x == 0) {
       if (!(mode & 8)) {
         const double d = fabs(st[MCG_S_E] - r4[0]);
@@ -775,7 +906,7 @@ __global__ __launch_bounds__(256) void k_measure(mcg_batch B, const int32_t* __r
   __syncthreads();
   if (mode & 4) {
     FrameCtx C{&G, G.coup, G.clsdat, s_red, s_frm, 0, (int)threadIdx.x, (int)blockDim.x};
-    measure_frame<NC, ST>(sp, C, B, st, row, frame, B.seed_lo, B.seed_hi ^ rp.serial, h);
+    measure_frame<NC>(sp, C, B, st, row, frame, B.seed_lo, B.seed_hi ^ rp.serial, h);
   }
   __syncthreads();
   if (threadIdx.x < MCG_NSTATE) B.state[(size_t)rep * MCG_NSTATE + threadIdx.x] = st[threadIdx.x];
@@ -787,11 +918,12 @@ __global__ __launch_bounds__(256) void k_measure(mcg_batch B, const int32_t* __r
 // ---------------------------------------------------------------------------
 __global__ void k_pt(mcg_batch B, const int32_t* __restrict__ ladder_row0, const int32_t* __restrict__ ladder_n,
                      int nladder, int64_t swap_index) {
-  const int L = blockIdx.x * blockDim.x + threadIdx.x;
+  const int L = blockIdx.x;
   if (L >= nladder) return;
   const int r0 = ladder_row0[L], nr = ladder_n[L];
   const int parity = (int)(swap_index & 1);
-  for (int t = parity; t + 1 < nr; t += 2) {
+  // the pairs of one parity are disjoint: one thread each
+  for (int t = parity + 2 * (int)threadIdx.x; t + 1 < nr; t += 2 * (int)blockDim.x) {
     const int ra = B.who[r0 + t], rb = B.who[r0 + t + 1];
     const double Ea = B.state[(size_t)ra * MCG_NSTATE + MCG_S_E], Eb = B.state[(size_t)rb * MCG_NSTATE + MCG_S_E];
     const double ba = B.row_beta[r0 + t], bb = B.row_beta[r0 + t + 1];
@@ -871,7 +1003,7 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
       for (int f = lo + tid; f < hi; f += blockDim.x) {
         const int o = __hip_atomic_load(mem + f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         float so[3];
-        spin_vec<NC, ST>(sp, o, G.cls, G.clsdat, so);
+        spin_vec<NC>(sp, o, G.cls, G.clsdat, so);
         const float sdo = -(so[0] * r0 + so[1] * r1 + so[2] * r2);
         for (int l = 0; l < G.zmax; ++l) {
           const uint32_t wd = G.link[(size_t)o * G.zmax + l];
@@ -885,7 +1017,7 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
             corr = 2.0f * corr;                            // add if J s s < 0, p = 1-exp(2 corr)
           } else {
             float st3[3];
-            spin_vec<NC, ST>(sp, t, G.cls, G.clsdat, st3);
+            spin_vec<NC>(sp, t, G.cls, G.clsdat, st3);
             const float sdt = -(st3[0] * r0 + st3[1] * r1 + st3[2] * r2);
             const float rJr = r0 * (J[0] * r0 + J[1] * r1 + J[2] * r2) + r1 * (J[3] * r0 + J[4] * r1 + J[5] * r2) +
                               r2 * (J[6] * r0 + J[7] * r1 + J[8] * r2);
@@ -917,7 +1049,7 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
     for (int f = tid; f < cnt; f += blockDim.x) {
       const int b = __hip_atomic_load(mem + f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       float sb[3];
-      spin_vec<NC, ST>(sp, b, G.cls, G.clsdat, sb);
+      spin_vec<NC>(sp, b, G.cls, G.clsdat, sb);
       if constexpr (NC == 1) {
         d += (double)(2.0f * h * sb[0]);
       } else {
@@ -929,7 +1061,7 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
           const float* J = G.coup + (wd >> 24) * 9;
           if ((wd >> 24) == 0u) continue;
           float stt[3];
-          spin_vec<NC, ST>(sp, t, G.cls, G.clsdat, stt);
+          spin_vec<NC>(sp, t, G.cls, G.clsdat, stt);
           const float sdt = -(stt[0] * r0 + stt[1] * r1 + stt[2] * r2);
           const float pt[3] = {stt[0] + sdt * r0, stt[1] + sdt * r1, stt[2] + sdt * r2};
           // r^T J p_t
@@ -975,7 +1107,7 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
           sp.set(b, 0, -sp.get(b, 0));
         } else {
           float sb[3];
-          spin_vec<NC, ST>(sp, b, G.cls, G.clsdat, sb);
+          spin_vec<NC>(sp, b, G.cls, G.clsdat, sb);
           const float a2 = -2.0f * (sb[0] * r0 + sb[1] * r1 + sb[2] * r2);
           float n0 = sb[0] + a2 * r0, n1 = sb[1] + a2 * r1, n2 = sb[2] + a2 * r2;
           float nn = n0 * n0 + n1 * n1 + n2 * n2;
@@ -1062,6 +1194,39 @@ struct RunCtx {
   std::vector<int32_t> hl_row0, hl_n;
 };
 
+// HIP-event bracketing of the sweep-kernel launches (plan->timing)
+struct LaunchTimer {
+  bool on = false;
+  std::vector<hipEvent_t> ev;
+  double launches = 0, sweeps = 0, updates = 0;
+  ~LaunchTimer() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+  int mark(hipStream_t s) {
+    if (!on) return 0;
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    ev.push_back(e);
+    HIPCHK(hipEventRecord(e, s));
+    return 0;
+  }
+  int finish(double* out) {
+    if (!on || !out) return 0;
+    double ms = 0;
+    for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+      HIPCHK(hipEventSynchronize(ev[i + 1]));
+      float t = 0;
+      HIPCHK(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
+      ms += t;
+    }
+    out[0] = ms;
+    out[1] = launches;
+    out[2] = sweeps;
+    out[3] = updates;
+    return 0;
+  }
+};
+
 int device_lds_max(size_t* out) {
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
@@ -1087,7 +1252,15 @@ int plan_lds(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool force_stream
   X.hstream.clear();
   X.hall.clear();
   for (int r = 0; r < B->nrep; ++r) X.hall.push_back(r);
-  const int block = P->block > 0 ? P->block : 256;
+  int block = P->block;
+  if (block <= 0) {
+    int maxcol = 0;
+    for (int gi = 0; gi < B->ngraph; ++gi) {
+      const mcg_graph& G = B->graphs_host[gi];
+      maxcol = std::max(maxcol, (G.n + G.ncol - 1) / std::max(G.ncol, 1));
+    }
+    block = maxcol >= 1024 ? 1024 : 256;
+  }
   if (block % 64 || block < 64 || block > 1024) return fail("block must be a multiple of 64 in [64,1024]");
   X.block = block;
   const int nwaves = block / 64;
@@ -1098,13 +1271,13 @@ int plan_lds(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool force_stream
     const mcg_graph& G = B->graphs_host[gi];
     const std::vector<int>& rs = by[gi];
     if (rs.empty()) continue;
-    const size_t per = align16((size_t)B->nc * G.n * ss);
+    const size_t per = lds_replica_bytes(B->nc, G.n, ss);
     int Rmax = 0;
     if (!force_stream) {
       for (int R = nwaves; R >= 1; --R) {
         if (nwaves % R) continue;
         if (P->reps_per_block > 0 && R > P->reps_per_block) continue;
-        if (lds_fixed_bytes(G.ncoup, G.ncls, nwaves, R) + R * per <= X.lds_max) {
+        if (lds_fixed_bytes(G.ncoup, G.ncls, G.npat, G.zmax, G.n, nwaves, R) + R * per <= X.lds_max) {
           Rmax = R;
           break;
         }
@@ -1141,7 +1314,8 @@ size_t lds_bytes_for(const mcg_batch* B, const RunCtx& X) {
   const int nwaves = X.block / 64;
   for (const WorkItem& w : X.hitems) {
     const mcg_graph& G = B->graphs_host[w.graph];
-    const size_t b = lds_fixed_bytes(G.ncoup, G.ncls, nwaves, w.nrep) + w.nrep * align16((size_t)NC * G.n * sizeof(ST));
+    const size_t b = lds_fixed_bytes(G.ncoup, G.ncls, G.npat, G.zmax, G.n, nwaves, w.nrep) +
+                     w.nrep * lds_replica_bytes(NC, G.n, sizeof(ST));
     mx = std::max(mx, b);
   }
   return mx;
@@ -1205,7 +1379,7 @@ int launch_fill_i32(int32_t* p, size_t n, int32_t v, hipStream_t s) {
 // planned sweep schedule for the streaming / plain paths
 template <int NC, typename ST>
 int stream_sweeps(const mcg_batch* B, const mcg_plan* P, RunCtx& X, int64_t k0, int64_t nsw, int64_t nthermal,
-                  int spf, bool plain, hipStream_t s) {
+                  int spf, bool plain, hipStream_t s, LaunchTimer& T) {
   if (X.hstream.empty()) return 0;
   int ncolmax = 0;
   const int maxn = max_colour_count(B, X.hstream, &ncolmax);
@@ -1216,11 +1390,18 @@ int stream_sweeps(const mcg_batch* B, const mcg_plan* P, RunCtx& X, int64_t k0, 
     const int64_t kk = P->sweep0 + k;
     for (int c = 0; c < ncolmax; ++c) {
       if (plain) {
-        for (int i = 0; i < nrs; ++i)
-          if (launch_pass<NC, ST>(B, dreps + i, 1, maxn, c, kk, s)) return -1;
-      } else if (launch_pass<NC, ST>(B, dreps, nrs, maxn, c, kk, s)) {
-        return -1;
+        for (int i = 0; i < nrs; ++i) {
+          if (T.mark(s) || launch_pass<NC, ST>(B, dreps + i, 1, maxn, c, kk, s) || T.mark(s)) return -1;
+          T.launches += 1;
+        }
+      } else {
+        if (T.mark(s) || launch_pass<NC, ST>(B, dreps, nrs, maxn, c, kk, s) || T.mark(s)) return -1;
+        T.launches += 1;
       }
+    }
+    if (T.on) {
+      T.sweeps += 1;
+      for (int r : X.hstream) T.updates += B->graphs_host[B->reps_host[r].graph].n;
     }
     const bool meas = k >= nthermal && ((k - nthermal + 1) % spf) == 0;
     const bool therm_end = (k + 1 == nthermal);
@@ -1290,7 +1471,7 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
       }
       if (P->pt_every > 0 && ((k + 1) % P->pt_every) == 0 && !X.hl_row0.empty()) {
         if (launch_measure<NC, ST>(B, (const int32_t*)X.all_reps.p, nall, 1 | 8, -1, s)) return -1;
-        hipLaunchKernelGGL(k_pt, dim3((unsigned)((X.hl_row0.size() + 63) / 64)), dim3(64), 0, s, *B,
+        hipLaunchKernelGGL(k_pt, dim3((unsigned)X.hl_row0.size()), dim3(64), 0, s, *B,
                            (const int32_t*)X.lad_row0.p, (const int32_t*)X.lad_n.p, (int)X.hl_row0.size(),
                            P->sweep0 + k);
         HIPCHK(hipGetLastError());
@@ -1299,21 +1480,30 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
     return 0;
   }
 
+  LaunchTimer T;
+  T.on = P->timing != nullptr;
+  double lds_updates_per_sweep = 0;
+  for (const WorkItem& w : X.hitems) lds_updates_per_sweep += (double)w.nrep * B->graphs_host[w.graph].n;
   const int64_t chunk = P->pt_every > 0 ? P->pt_every : total;
   for (int64_t k = 0; k < total;) {
     const int64_t ns = std::min<int64_t>(chunk, total - k);
     SweepArgs a{k, P->sweep0, (int32_t)ns, (int32_t)nth, spf, P->frame0};
-    if (launch_lds<NC, ST>(B, X, a, s)) return -1;
-    if (stream_sweeps<NC, ST>(B, P, X, k, ns, nth, spf, plain, s)) return -1;
+    if (!X.hitems.empty()) {
+      if (T.mark(s) || launch_lds<NC, ST>(B, X, a, s) || T.mark(s)) return -1;
+      T.launches += 1;
+      T.sweeps += (double)ns;
+      T.updates += lds_updates_per_sweep * (double)ns;
+    }
+    if (stream_sweeps<NC, ST>(B, P, X, k, ns, nth, spf, plain, s, T)) return -1;
     k += ns;
     if (P->pt_every > 0 && (k % P->pt_every) == 0 && !X.hl_row0.empty()) {
-      hipLaunchKernelGGL(k_pt, dim3((unsigned)((X.hl_row0.size() + 63) / 64)), dim3(64), 0, s, *B,
+      hipLaunchKernelGGL(k_pt, dim3((unsigned)X.hl_row0.size()), dim3(64), 0, s, *B,
                          (const int32_t*)X.lad_row0.p, (const int32_t*)X.lad_n.p, (int)X.hl_row0.size(),
                          P->sweep0 + k - 1);
       HIPCHK(hipGetLastError());
     }
   }
-  return 0;
+  return T.finish(P->timing);
 }
 
 template <int NC, typename ST>

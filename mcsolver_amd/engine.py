@@ -55,6 +55,10 @@ class GraphTables:
     gmax: int
     rrank: np.ndarray
     colour: np.ndarray
+    pos: Optional[np.ndarray] = None
+    pat: Optional[np.ndarray] = None
+    pattab: Optional[np.ndarray] = None
+    npat: int = 0
     dev: Optional[dict] = None
     struct: Optional[L.Graph] = None
 
@@ -166,8 +170,23 @@ def compile_graph(nc: int, nbr: np.ndarray, J: np.ndarray, S: np.ndarray, D: np.
     garr = -np.ones((max(len(groups), 1), gmax), np.int32)
     for k, g in enumerate(groups):
         garr[k, :len(g)] = g
+    # colour-permuted link patterns for the LDS-resident kernel
+    pos = np.empty(n, np.int32)
+    pos[order] = np.arange(n, dtype=np.int32)
+    site_of = order.astype(np.int64)
+    delta = pos[tgt[site_of]].astype(np.int64) - np.arange(n)[:, None]
+    keyrows = np.concatenate([delta, cid[site_of].astype(np.int64), cls[site_of].astype(np.int64)[:, None]], 1)
+    prow, pinv = np.unique(keyrows, axis=0, return_inverse=True)
+    if len(prow) <= 256 and n < (1 << 23):
+        npat = len(prow)
+        words = (prow[:, :zmax] & 0xFFFFFF) | (prow[:, zmax:2 * zmax] << 24)
+        pattab = np.concatenate([words, prow[:, 2 * zmax:]], 1).astype(np.uint32).view(np.int32).reshape(-1)
+        pat = pinv.reshape(-1).astype(np.uint8)
+    else:
+        npat, pattab, pat = 0, np.zeros(1, np.int32), np.zeros(n, np.uint8)
     return GraphTables(n, zmax, link.reshape(-1), uniq.astype(np.float32).reshape(-1), cls, cu.reshape(-1),
-                       col_ptr, order, pairs, circ, rorb, rclu, rlk.reshape(-1), garr, gmax, rrank, colour)
+                       col_ptr, order, pairs, circ, rorb, rclu, rlk.reshape(-1), garr, gmax, rrank, colour,
+                       pos=pos, pat=pat, pattab=pattab, npat=npat)
 
 
 def _dev(a: np.ndarray, device) -> torch.Tensor:
@@ -183,7 +202,8 @@ def upload_graph(g: GraphTables, device) -> None:
     if g.dev is not None:
         return
     d = {k: _dev(getattr(g, k), device) for k in ("link", "coup", "cls", "clsdat", "col_ptr", "col_site", "pairs",
-                                                  "circ", "rorb", "rclu", "rlink", "groups", "rrank")}
+                                                  "circ", "rorb", "rclu", "rlink", "groups", "rrank", "pos", "pat",
+                                                  "pattab")}
     g.dev = d
     s = L.Graph()
     s.n, s.zmax, s.ncol = g.n, g.zmax, g.ncol
@@ -192,6 +212,7 @@ def upload_graph(g: GraphTables, device) -> None:
     s.nclu = g.rclu.shape[1] if len(g.rorb) else 0
     s.ngroup = 0 if (g.groups < 0).all() else g.groups.shape[0]
     s.gmax = g.gmax
+    s.npat = g.npat
     for k, t in d.items():
         setattr(s, k, t.data_ptr())
     g.struct = s

@@ -11,9 +11,11 @@
  *
  * Engine conventions restated here:
  *   key = (seed_lo, seed_hi ^ serial)
- *   Metropolis draw for site i at sweep k: ctr = (i, k_lo, k_hi, 0)
- *     Heisenberg: r_z = 2 u(x0) - 1, phi from x1, accept u(x2)
- *     XY:         phi from x1, accept u(x2);   Ising: accept u(x2)
+ *   Metropolis: colour c, positions q*G..q*G+G-1 of its site list (sites in
+ *     site order) share the draw ctr = (q, k_lo, k_hi, c << 8), G = 1
+ *     (Heisenberg: r_z = 2 u(x0) - 1, phi from x1, accept u(x2)),
+ *     G = 2 (XY: site g takes phi from x_{2g}, accept u(x_{2g+1})),
+ *     G = 4 (Ising: site g accepts with u(x_g))
  *   u(x) = (x >> 8) * 2^-24
  *   Wolff cluster c: ctr (0, c_lo, c_hi, 2): seed = x0*n >> 32, direction
  *     from x1 (XY phi; Heisenberg r_z) and x2 (Heisenberg phi), accept x3>>8
@@ -152,12 +154,12 @@ static void cb_free(cb_graph *g) {
 
 static float stor(const cb_graph *g, float v) { return g->store == 1 ? f16r(v) : v; }
 
-/* spins: [nc][n] floats for one replica (Ising: sigma = +-1) */
-static void cb_site(const cb_graph *g, float *sp, long i, float beta, float h, uint64_t k, uint32_t k0, uint32_t k1,
+/* spins: [nc][n] floats for one replica (Ising: sigma = +-1).
+ * wa: r_z word (Heisenberg), wb: angle word, wc: acceptance word */
+static void cb_site(const cb_graph *g, float *sp, long i, float beta, float h, uint32_t wa, uint32_t wb, uint32_t wc,
                     double *dE, double *dM) {
   long n = g->n;
   int nc = g->nc;
-  ref_u4 x = ref_philox((uint32_t)i, (uint32_t)k, (uint32_t)(k >> 32), 0u, k0, k1);
   const float S = g->S[i];
   if (nc == 1) {
     float f = 0.0f;
@@ -171,7 +173,7 @@ static void cb_site(const cb_graph *g, float *sp, long i, float beta, float h, u
     float t = f - h;
     float de = (-2.0f * s) * t;
     float db = de * beta;
-    if (db <= 0.0f || uu(x.v[2]) < cexp_neg(-db)) {
+    if (db <= 0.0f || uu(wc) < cexp_neg(-db)) {
       sp[i] = -sg;
       *dE += de;
       dM[0] += -2.0f * s;
@@ -197,14 +199,14 @@ static void cb_site(const cb_graph *g, float *sp, long i, float beta, float h, u
   }
   float rx, ry, rz = 0.0f;
   if (nc == 3) {
-    rz = fmaf(uu(x.v[0]), 2.0f, -1.0f);
+    rz = fmaf(uu(wa), 2.0f, -1.0f);
     float rho = sqrtf(fmaxf(fmaf(-rz, rz, 1.0f), 0.0f));
     float c, sn;
-    csincos(x.v[1], &c, &sn);
+    csincos(wb, &c, &sn);
     rx = rho * c;
     ry = rho * sn;
   } else {
-    csincos(x.v[1], &rx, &ry);
+    csincos(wb, &rx, &ry);
   }
   float p = s0 * rx;
   p = fmaf(s1, ry, p);
@@ -228,7 +230,7 @@ static void cb_site(const cb_graph *g, float *sp, long i, float beta, float h, u
   }
   de = fmaf(-h, nc == 3 ? d2 : d0, de);
   float db = de * beta;
-  if (db <= 0.0f || uu(x.v[2]) < cexp_neg(-db)) {
+  if (db <= 0.0f || uu(wc) < cexp_neg(-db)) {
     float nn = n0 * n0;
     nn = fmaf(n1, n1, nn);
     if (nc == 3) nn = fmaf(n2, n2, nn);
@@ -271,8 +273,24 @@ int mcref_cb_sweeps(int nc, int store, long n, int zmax, const int *nbr, const d
     uint32_t k0 = seed_lo, k1 = seed_hi ^ serial[r];
     for (long s = 0; s < nsweeps; s++) {
       uint64_t k = (uint64_t)(sweep0 + s);
-      for (int c = 0; c < ncol; c++)
-        for (long q = ptr[c]; q < ptr[c + 1]; q++) cb_site(&g, sp, order[q], beta[r], h[r], k, k0, k1, dE + r, dM + 3 * r);
+      for (int c = 0; c < ncol; c++) {
+        /* one Philox draw per group of G consecutive colour-list positions:
+           G = 4 (Ising), 2 (XY), 1 (Heisenberg) */
+        long cnt = ptr[c + 1] - ptr[c];
+        int G = nc == 1 ? 4 : nc == 2 ? 2 : 1;
+        for (long q = 0; q * G < cnt; q++) {
+          ref_u4 x = ref_philox((uint32_t)q, (uint32_t)k, (uint32_t)(k >> 32), (uint32_t)c << 8, k0, k1);
+          for (int gg = 0; gg < G && q * G + gg < cnt; gg++) {
+            long i = order[ptr[c] + q * G + gg];
+            if (nc == 3)
+              cb_site(&g, sp, i, beta[r], h[r], x.v[0], x.v[1], x.v[2], dE + r, dM + 3 * r);
+            else if (nc == 2)
+              cb_site(&g, sp, i, beta[r], h[r], 0u, x.v[2 * gg], x.v[2 * gg + 1], dE + r, dM + 3 * r);
+            else
+              cb_site(&g, sp, i, beta[r], h[r], 0u, 0u, x.v[gg], dE + r, dM + 3 * r);
+          }
+        }
+      }
     }
   }
   free(order);

@@ -178,3 +178,29 @@ def test_parallel_tempering_bitexact(gpu):
     assert np.array_equal(b.rung.cpu().numpy(), rung)
     assert np.array_equal(b.who.cpu().numpy(), who)
     assert np.array_equal(b.pt_acc.cpu().numpy(), acc)
+
+
+def test_lds_without_link_patterns(gpu):
+    """Irregular graph (no translation patterns): LDS kernel on the global
+    link table, still bit-exact."""
+    rng = np.random.default_rng(2)
+    n, z = 300, 5
+    nbr = rng.integers(0, n, (n, z))
+    nbr[rng.random((n, z)) < 0.2] = -1
+    J = rng.normal(size=(n, z, 9)) * 0.5
+    S = np.where(rng.random(n) < 0.5, 1.0, 1.5)
+    D = rng.normal(size=(n, 3)) * 0.1
+    g = compile_graph(3, nbr, J, S, D)
+    assert g.npat == 0
+    betas = [0.7, 1.3]
+    init = []
+    for _ in betas:
+        v = rng.normal(size=(n, 3))
+        init.append(v / np.linalg.norm(v, axis=1, keepdims=True) * S[:, None])
+    b = Batch(3, [g], [ReplicaSpec(0, k) for k in range(2)], betas, [0.2, 0.2], init, seed=SEED)
+    b.init()
+    b.sweeps(0, 6, kernel=L.KERNEL_LDS)
+    got = _gpu_spins(b, 2)
+    ref = _cpu_spins(3, init, "f32")
+    mcref.cb_sweeps(3, 0, nbr, J, S, D, g.colour, betas, [0.2, 0.2], np.arange(2, dtype=np.uint32), SEED, 0, 6, ref)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))

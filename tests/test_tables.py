@@ -1,0 +1,91 @@
+"""The table compiler: device tables decode back to the per-site links."""
+import numpy as np
+import pytest
+
+from mcsolver_amd import lattice as lat
+from mcsolver_amd.engine import compile_graph
+
+from cases import _honeycomb, _triangular_dmi
+
+ROWMAJOR = [0, 3, 4, 6, 1, 5, 7, 8, 2]
+
+
+def _check(nc, L_):
+    J = L_.J if nc > 1 else L_.J[:, :, :1]
+    g = compile_graph(nc, L_.nbr, J, L_.S, L_.D)
+    n, z = L_.nbr.shape
+    link = g.link.reshape(n, z).astype(np.int64)
+    nbr = link & 0xFFFFFF
+    cid = link >> 24
+    coup = g.coup.reshape(-1, 9)
+    valid = L_.nbr >= 0
+    assert np.array_equal(nbr[valid], L_.nbr[valid])
+    assert (nbr[~valid] == np.nonzero(~valid)[0]).all() if (~valid).any() else True
+    assert np.all(coup[0] == 0)
+    for i in range(n):
+        for l in range(z):
+            if not valid[i, l]:
+                assert cid[i, l] == 0
+                continue
+            if nc == 1:
+                assert coup[cid[i, l], 0] == np.float32(L_.J[i, l, 0] * L_.S[L_.nbr[i, l]])
+                assert coup[cid[i, l], 1] == np.float32(L_.J[i, l, 0])
+            else:
+                want = L_.J[i, l, ROWMAJOR].astype(np.float32)
+                if nc == 2:  # XY keeps xx, xy, yx, yy (xyLib.c:150-153)
+                    want[[2, 5, 6, 7, 8]] = 0
+                assert np.array_equal(coup[cid[i, l]], want)
+    # proper colouring
+    col = g.colour
+    for i in range(n):
+        for l in range(z):
+            j = L_.nbr[i, l]
+            if j >= 0 and j != i:
+                assert col[i] != col[j]
+    assert sorted(g.col_site.tolist()) == list(range(n))
+    assert g.col_ptr[-1] == n
+    # class table
+    cd = g.clsdat.reshape(-1, 4)
+    assert np.array_equal(cd[g.cls, 0], L_.S.astype(np.float32))
+    assert np.array_equal(cd[g.cls, 1:], L_.D.astype(np.float32))
+    # link patterns in colour-permuted positions
+    if g.npat:
+        pt = g.pattab.view(np.uint32).reshape(g.npat, z + 1).astype(np.int64)
+        for p in range(n):
+            i = g.col_site[p]
+            row = pt[g.pat[p]]
+            d = ((row[:z] & 0xFFFFFF) ^ 0x800000) - 0x800000
+            assert np.array_equal(g.col_site[p + d], nbr[i]), p
+            assert np.array_equal(row[:z] >> 24, cid[i])
+            assert row[z] == g.cls[i]
+        assert np.array_equal(g.pos[g.col_site], np.arange(n))
+    return g
+
+
+@pytest.mark.parametrize("mk", [lambda: (1, lat.square(8, ising=True)), lambda: (2, lat.square(6, J=-1.0)),
+                                lambda: (3, _honeycomb(4, 1)), lambda: (3, _triangular_dmi(6)),
+                                lambda: (3, lat.cubic(4))])
+def test_tables_roundtrip(mk):
+    nc, L_ = mk()
+    g = _check(nc, L_)
+    assert g.npat > 0 and g.npat <= 64
+
+
+def test_square_lattice_is_checkerboard():
+    g = compile_graph(2, lat.square(8).nbr, lat.square(8).J, np.ones(64), np.zeros((64, 3)))
+    assert g.ncol == 2
+    x, y = np.divmod(np.arange(64), 8)
+    assert len(set(zip(g.colour, (x + y) % 2))) == 2
+
+
+def test_random_graph_has_no_patterns():
+    rng = np.random.default_rng(0)
+    n, z = 400, 4
+    nbr = rng.integers(0, n, (n, z))
+    g = compile_graph(3, nbr, -np.ones((n, z, 9)), np.ones(n), np.zeros((n, 3)))
+    assert g.npat == 0
+
+
+def test_bad_indices_rejected():
+    with pytest.raises(ValueError):
+        compile_graph(3, np.array([[1], [5]]), np.zeros((2, 1, 9)), np.ones(2), np.zeros((2, 3)))
