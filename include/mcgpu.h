@@ -69,7 +69,7 @@ typedef struct mcg_graph {
   int32_t npat;            /* link patterns (0 = none: LDS kernel reads link[]) */
   const uint32_t *link;    /* [n*zmax] */
   const float *coup;       /* [ncoup*9] row-major 3x3 (E = s_i^T J s_j), unscaled */
-  const uint8_t *cls;      /* [n] site class */
+  const int32_t *cls;      /* [n] site class */
   const float *clsdat;     /* [ncls*4] : S (signed spin length), Dx, Dy, Dz */
   const int32_t *col_ptr;  /* [ncol+1] */
   const int32_t *col_site; /* [n] sites grouped by colour */
@@ -87,6 +87,9 @@ typedef struct mcg_graph {
   const int32_t *pos;      /* [n] */
   const uint8_t *pat;      /* [n] */
   const int32_t *pattab;   /* [npat*(zmax+1)] */
+  const int32_t *lcid;     /* [n*zmax] coupling index per link when there are more
+                              than 256 distinct tensors (link words then carry
+                              coupling 0); NULL otherwise */
 } mcg_graph;
 /* Ising couplings: coup[9c+0] = J*S_target (folded, used by the sweep and the
  * energy), coup[9c+1] = J (raw, used by the block-spin energy). */

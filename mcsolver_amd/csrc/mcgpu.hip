@@ -61,31 +61,57 @@ extern "C" int mcg_version(void) { return 1; }
 // ---------------------------------------------------------------------------
 // spin storage
 // ---------------------------------------------------------------------------
+// pointers into HBM with their address space spelled out, so that the
+// compiler emits global (not flat) memory instructions
+#define GAS __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ GAS T* gas(T* p) {
+  return (GAS T*)p;
+}
+template <typename T>
+__device__ __forceinline__ const GAS T* gas(const T* p) {
+  return (const GAS T*)p;
+}
+
 template <typename ST>
 struct SIO;
 template <>
 struct SIO<float> {
-  static __device__ __forceinline__ float ld(const float* p) { return *p; }
-  static __device__ __forceinline__ void st(float* p, float v) { *p = v; }
+  template <class P>
+  static __device__ __forceinline__ float ld(P p) { return *p; }
+  template <class P>
+  static __device__ __forceinline__ void st(P p, float v) { *p = v; }
 };
 template <>
 struct SIO<__half> {
   static __device__ __forceinline__ float ld(const __half* p) { return __half2float(*p); }
   static __device__ __forceinline__ void st(__half* p, float v) { *p = __float2half_rn(v); }
+  // HBM: move the 16-bit pattern as a scalar
+  static __device__ __forceinline__ float ld(const GAS __half* p) {
+    return __half2float(__ushort_as_half(*(const GAS unsigned short*)p));
+  }
+  static __device__ __forceinline__ void st(GAS __half* p, float v) {
+    *(GAS unsigned short*)p = __half_as_ushort(__float2half_rn(v));
+  }
 };
 template <>
 struct SIO<int8_t> {
-  static __device__ __forceinline__ float ld(const int8_t* p) { return (float)*p; }
-  static __device__ __forceinline__ void st(int8_t* p, float v) { *p = (int8_t)v; }
+  template <class P>
+  static __device__ __forceinline__ float ld(P p) { return (float)*p; }
+  template <class P>
+  static __device__ __forceinline__ void st(P p, float v) { *p = (int8_t)v; }
 };
 
-template <typename ST>
+// one replica's spins; PT is ST* (LDS, inferred) or GAS ST* (HBM)
+template <typename ST, typename PT = ST*>
 struct Spins {
-  ST* p;
+  PT p;
   int n;
   __device__ __forceinline__ float get(int i, int c) const { return SIO<ST>::ld(p + (size_t)c * n + i); }
   __device__ __forceinline__ void set(int i, int c, float v) const { SIO<ST>::st(p + (size_t)c * n + i, v); }
 };
+template <typename ST>
+using GSpins = Spins<ST, GAS ST*>;
 
 // colour-permuted replica (LDS): site i lives at position pos[i]
 template <typename ST>
@@ -100,9 +126,10 @@ struct PSpins {
 // neighbour accessors: the global link table (site order) or a pattern row
 // (colour-permuted positions, LDS)
 struct GlobalLinks {
-  const uint32_t* lk;
+  const GAS uint32_t* lk;
+  const GAS int32_t* lc;  // per-link coupling indices (wide graphs) or nullptr
   __device__ __forceinline__ int nbr(int l) const { return (int)(lk[l] & 0xFFFFFFu); }
-  __device__ __forceinline__ uint32_t cid(int l) const { return lk[l] >> 24; }
+  __device__ __forceinline__ uint32_t cid(int l) const { return lc ? (uint32_t)lc[l] : lk[l] >> 24; }
 };
 struct PatLinks {
   const int32_t* row;
@@ -124,10 +151,10 @@ static constexpr double REF_PI = 3.1415926535;  // heisenbergLib.c:6
 // renormalised to |S|.
 // Random words: w_a = r_z (Heisenberg), w_b = phi (XY, Heisenberg),
 // w_c = acceptance.
-template <int NC, typename SV, typename LK>
-__device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, int zmax, const float* coup, int ci,
-                                           const float* clsdat, float beta, float h, uint32_t w_a, uint32_t w_b,
-                                           uint32_t w_c, double& dEacc, double* dMacc) {
+template <int NC, typename SV, typename LK, typename CP>
+__device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, int zmax, CP coup, int ci, CP clsdat,
+                                           float beta, float h, uint32_t w_a, uint32_t w_b, uint32_t w_c,
+                                           double& dEacc, double* dMacc) {
   const float S = clsdat[ci * 4 + 0];
   if constexpr (NC == 1) {
     float f = 0.0f;
@@ -149,7 +176,7 @@ __device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, in
     float H0 = 0.0f, H1 = 0.0f, H2 = 0.0f;
     for (int l = 0; l < zmax; ++l) {
       const int j = lk.nbr(l);
-      const float* J = coup + lk.cid(l) * 9;
+      const CP J = coup + lk.cid(l) * 9;
       const float t0 = sp.get(j, 0), t1 = sp.get(j, 1);
       if constexpr (NC == 3) {
         const float t2 = sp.get(j, 2);
@@ -221,12 +248,13 @@ struct SitesPerDraw {
   static constexpr int G = NC == 1 ? 4 : NC == 2 ? 2 : 1;
 };
 
-template <int NC, typename ST>
-__device__ __forceinline__ void metro_group(const Spins<ST>& sp, const int32_t* __restrict__ cs, int cnt, int q,
-                                            int colour, const uint32_t* __restrict__ link, int zmax,
-                                            const float* coup, const uint8_t* __restrict__ cls, const float* clsdat,
-                                            float beta, float h, uint64_t sweep, uint32_t key0, uint32_t key1,
-                                            double& dEacc, double* dMacc) {
+template <int NC, typename SV, typename CP>
+__device__ __forceinline__ void metro_group(const SV& sp, const GAS int32_t* __restrict__ cs, int cnt, int q,
+                                            int colour, const GAS uint32_t* __restrict__ link,
+                                            const GAS int32_t* __restrict__ lcid, int zmax, CP coup,
+                                            const GAS int32_t* __restrict__ cls, CP clsdat, float beta, float h,
+                                            uint64_t sweep, uint32_t key0, uint32_t key1, double& dEacc,
+                                            double* dMacc) {
   constexpr int G = SitesPerDraw<NC>::G;
   const u32x4 x = philox4x32_10(
       u32x4{(uint32_t)q, (uint32_t)sweep, (uint32_t)(sweep >> 32), TAG_METRO | ((uint32_t)colour << 8)}, key0, key1);
@@ -236,7 +264,7 @@ __device__ __forceinline__ void metro_group(const Spins<ST>& sp, const int32_t* 
     const int p = q * G + g;
     if (p < cnt) {
       const int i = cs[p];
-      const GlobalLinks lk{link + (size_t)i * zmax};
+      const GlobalLinks lk{link + (size_t)i * zmax, lcid ? lcid + (size_t)i * zmax : nullptr};
       const uint32_t wa = NC == 3 ? w[0] : 0u;
       const uint32_t wb = NC == 3 ? w[1] : NC == 2 ? w[2 * g] : 0u;
       const uint32_t wc = NC == 3 ? w[2] : NC == 2 ? w[2 * g + 1] : w[g];
@@ -267,14 +295,15 @@ __device__ __forceinline__ void metro_group_pat(const Spins<ST>& sp, int b, int 
       const uint32_t wa = NC == 3 ? w[0] : 0u;
       const uint32_t wb = NC == 3 ? w[1] : NC == 2 ? w[2 * g] : 0u;
       const uint32_t wc = NC == 3 ? w[2] : NC == 2 ? w[2 * g + 1] : w[g];
-      metro_core<NC>(sp, p, lk, zmax, coup, row[zmax], clsdat, beta, h, wa, wb, wc, dEacc, dMacc);
+      metro_core<NC, Spins<ST>, PatLinks, const float*>(sp, p, lk, zmax, coup, row[zmax], clsdat, beta, h, wa, wb, wc,
+                                                         dEacc, dMacc);
     }
   }
 }
 
 // spin vector of site i (Ising: sigma*S)
 template <int NC, typename SV>
-__device__ __forceinline__ void spin_vec(const SV& sp, int i, const uint8_t* __restrict__ cls, const float* clsdat,
+__device__ __forceinline__ void spin_vec(const SV& sp, int i, const int32_t* __restrict__ cls, const float* clsdat,
                                          float* v) {
   if constexpr (NC == 1) {
     v[0] = sp.get(i, 0) * clsdat[cls[i] * 4];
@@ -288,26 +317,22 @@ __device__ __forceinline__ void spin_vec(const SV& sp, int i, const uint8_t* __r
 
 // per-site energy e_i (FP32): 1/2 s_i.H_i + onsite - h s_axis
 template <int NC, typename SV>
-__device__ __forceinline__ float site_energy(const SV& sp, int i, const uint32_t* __restrict__ link, int zmax,
-                                             const float* coup, const uint8_t* __restrict__ cls, const float* clsdat,
-                                             float h) {
+__device__ __forceinline__ float site_energy(const SV& sp, int i, const uint32_t* __restrict__ link,
+                                             const int32_t* __restrict__ lcid, int zmax, const float* coup,
+                                             const int32_t* __restrict__ cls, const float* clsdat, float h) {
   const int ci = cls[i];
-  const uint32_t* lk = link + (size_t)i * zmax;
+  const GlobalLinks lk{gas(link) + (size_t)i * zmax, lcid ? gas(lcid) + (size_t)i * zmax : nullptr};
   if constexpr (NC == 1) {
     float f = 0.0f;
-    for (int l = 0; l < zmax; ++l) {
-      const uint32_t w = lk[l];
-      f = fmaf(coup[(w >> 24) * 9], sp.get((int)(w & 0xFFFFFFu), 0), f);
-    }
+    for (int l = 0; l < zmax; ++l) f = fmaf(coup[lk.cid(l) * 9], sp.get(lk.nbr(l), 0), f);
     const float s = sp.get(i, 0) * clsdat[ci * 4];
     return fmaf(0.5f * s, f, -h * s);
   } else {
     const float s0 = sp.get(i, 0), s1 = sp.get(i, 1), s2 = NC == 3 ? sp.get(i, 2) : 0.0f;
     float H0 = 0.0f, H1 = 0.0f, H2 = 0.0f;
     for (int l = 0; l < zmax; ++l) {
-      const uint32_t w = lk[l];
-      const int j = (int)(w & 0xFFFFFFu);
-      const float* J = coup + (w >> 24) * 9;
+      const int j = lk.nbr(l);
+      const float* J = coup + lk.cid(l) * 9;
       const float t0 = sp.get(j, 0), t1 = sp.get(j, 1), t2 = NC == 3 ? sp.get(j, 2) : 0.0f;
       H0 = fmaf(J[0], t0, H0); H0 = fmaf(J[1], t1, H0); H0 = fmaf(J[2], t2, H0);
       H1 = fmaf(J[3], t0, H1); H1 = fmaf(J[4], t1, H1); H1 = fmaf(J[5], t2, H1);
@@ -496,7 +521,10 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
       for (int l = 0; l < G.zmax; ++l) {
         const uint32_t wd = G.rlink[(size_t)ri * G.zmax + l];
         const int t = (int)(wd & 0xFFFFFFu);
-        const float* J = C.coup + (wd >> 24) * 9;
+        // wide graphs: the block-spin link takes the coupling of the site's
+        // link l; a set high byte marks an absent link
+        if (G.lcid && (wd >> 24)) continue;
+        const float* J = C.coup + (G.lcid ? G.lcid[(size_t)j * G.zmax + l] : (int)(wd >> 24)) * 9;
         float mt[3];
         majority<NC>(sp, G, C.clsdat, t, frame, key0, key1, mt);
         if constexpr (NC == 1) {
@@ -650,7 +678,7 @@ __device__ void recompute(const SV& sp, const mcg_graph& G, const float* coup, c
   double v[RK];
   for (int k = 0; k < RK; ++k) v[k] = 0.0;
   for (int i = lt; i < G.n; i += TPR) {
-    v[0] += (double)site_energy<NC>(sp, i, G.link, G.zmax, coup, G.cls, clsdat, h);
+    v[0] += (double)site_energy<NC>(sp, i, G.link, G.lcid, G.zmax, coup, G.cls, clsdat, h);
     float s[3];
     spin_vec<NC>(sp, i, G.cls, clsdat, s);
     v[1] += s[0];
@@ -682,9 +710,9 @@ static constexpr int FRM = 96;  // frame scratch doubles per group
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-__host__ __device__ inline size_t lds_fixed_bytes(int ncoup, int ncls, int npat, int zmax, int n, int nwaves,
-                                                  int R) {
-  size_t b = align16((size_t)(ncoup * 9 + ncls * 4) * 4);
+__host__ __device__ inline size_t lds_fixed_bytes(int ncoup, int ncls, int ncol, int npat, int zmax, int n,
+                                                  int nwaves, int R) {
+  size_t b = align16((size_t)(ncoup * 9 + ncls * 4) * 4) + align16((size_t)(ncol + 1) * 4);
   if (npat > 0) b += align16((size_t)npat * (zmax + 1) * 4) + align16((size_t)n);
   return b + align16((size_t)nwaves * RK * 8) + align16((size_t)R * (MCG_NSTATE + FRM) * 8);
 }
@@ -727,6 +755,56 @@ __device__ __forceinline__ void lds_sweep_tail(const SV& sv, const mcg_graph& G,
   __syncthreads();
 }
 
+// sweep loop of one LDS block; MODE 0: pattern links (colour-permuted
+// replica), 1: global link table with LDS-staged couplings/classes, 2:
+// global link table with couplings/classes read from HBM (> 256 tensors or
+// classes).  Each mode is its own instantiation so that every table pointer
+// has one address space.
+template <int NC, typename ST, int MODE>
+__device__ __forceinline__ void lds_sweeps(const mcg_graph& G, const mcg_batch& B, const SweepArgs& a,
+                                           const Spins<ST>& sp, const PSpins<ST>& psp, const float* coup,
+                                           const float* clsdat, const int32_t* s_colptr, const uint8_t* s_pid,
+                                           const int32_t* s_pat, double* s_red, double* st, FrameCtx& C, int g,
+                                           int lt, int TPR, int row, float beta, float h, uint32_t key0,
+                                           uint32_t key1) {
+  constexpr int GD = SitesPerDraw<NC>::G;
+  const int zmax = G.zmax;
+  double dE = 0.0, dM[3] = {0.0, 0.0, 0.0};
+  for (int s = 0; s < a.nsweeps; ++s) {
+    const int64_t k = a.k_begin + s;
+    const uint64_t kk = (uint64_t)(a.philox_base + k);
+    for (int c = 0; c < G.ncol; ++c) {
+      const int b = s_colptr[c], cnt = s_colptr[c + 1] - b;
+      const int ngrp = (cnt + GD - 1) / GD;
+      if constexpr (MODE == 0) {
+        for (int q = lt; q < ngrp; q += TPR)
+          metro_group_pat<NC, ST>(sp, b, cnt, q, c, s_pid, s_pat, zmax, coup, clsdat, beta, h, kk, key0, key1, dE,
+                                  dM);
+      } else if constexpr (MODE == 1) {
+        for (int q = lt; q < ngrp; q += TPR)
+          metro_group<NC>(sp, gas(G.col_site) + b, cnt, q, c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, coup,
+                          gas(G.cls), clsdat, beta, h, kk, key0, key1, dE, dM);
+      } else {
+        for (int q = lt; q < ngrp; q += TPR)
+          metro_group<NC>(sp, gas(G.col_site) + b, cnt, q, c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax,
+                          gas(coup), gas(G.cls), gas(clsdat), beta, h, kk, key0, key1, dE, dM);
+      }
+      __syncthreads();
+    }
+    const bool meas = k >= a.nthermal && ((k - a.nthermal + 1) % a.spf) == 0;
+    const bool therm_end = (k + 1 == a.nthermal);
+    const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
+    if (meas || therm_end || rec || s == a.nsweeps - 1) {
+      if constexpr (MODE == 0)
+        lds_sweep_tail<NC, ST>(psp, G, B, coup, clsdat, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end, rec,
+                               dE, dM, key0, key1);
+      else
+        lds_sweep_tail<NC, ST>(sp, G, B, coup, clsdat, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end, rec,
+                               dE, dM, key0, key1);
+    }
+  }
+}
+
 template <int NC, typename ST>
 __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __restrict__ items,
                                               const int32_t* __restrict__ item_reps, SweepArgs a) {
@@ -739,12 +817,17 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   const int nwaves = blockDim.x / 64;
   const bool perm = G.npat > 0;
   const int n = G.n, zmax = G.zmax;
+  const bool wide = G.lcid != nullptr || G.ncls > 256;
+  const int ncl = wide ? 0 : G.ncoup;  // couplings staged in LDS
+  const int ncs = wide ? 0 : G.ncls;   // site classes staged in LDS
 
   extern __shared__ __align__(16) unsigned char smem[];
   size_t off = 0;
   float* s_coup = (float*)smem;
-  float* s_cls = s_coup + G.ncoup * 9;
-  off += align16((size_t)(G.ncoup * 9 + G.ncls * 4) * 4);
+  float* s_clsl = s_coup + ncl * 9;
+  off += align16((size_t)(ncl * 9 + ncs * 4) * 4);
+  int32_t* s_colptr = (int32_t*)(smem + off);
+  off += align16((size_t)(G.ncol + 1) * 4);
   int32_t* s_pat = (int32_t*)(smem + off);
   if (perm) off += align16((size_t)G.npat * (zmax + 1) * 4);
   uint8_t* s_pid = (uint8_t*)(smem + off);
@@ -756,8 +839,9 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   off += align16((size_t)R * (MCG_NSTATE + FRM) * 8);
   ST* s_spin = (ST*)(smem + off);
 
-  for (int t = threadIdx.x; t < G.ncoup * 9; t += blockDim.x) s_coup[t] = G.coup[t];
-  for (int t = threadIdx.x; t < G.ncls * 4; t += blockDim.x) s_cls[t] = G.clsdat[t];
+  for (int t = threadIdx.x; t < ncl * 9; t += blockDim.x) s_coup[t] = G.coup[t];
+  for (int t = threadIdx.x; t < ncs * 4; t += blockDim.x) s_clsl[t] = G.clsdat[t];
+  for (int t = threadIdx.x; t <= G.ncol; t += blockDim.x) s_colptr[t] = G.col_ptr[t];
   if (perm) {
     for (int t = threadIdx.x; t < G.npat * (zmax + 1); t += blockDim.x) s_pat[t] = G.pattab[t];
     for (int t = threadIdx.x; t < n; t += blockDim.x) s_pid[t] = G.pat[t];
@@ -780,40 +864,20 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   const uint32_t key0 = B.seed_lo, key1 = B.seed_hi ^ rp.serial;
   const int row = rp.row0 + B.rung[rep];
   const float beta = B.row_beta[row], h = B.row_h[row];
-  double dE = 0.0, dM[3] = {0.0, 0.0, 0.0};
-  FrameCtx C{&G, s_coup, s_cls, s_red, s_frm + g * FRM, g, lt, TPR};
   const Spins<ST> sp{my, n};
   const PSpins<ST> psp{my, n, G.pos};
-  constexpr int GD = SitesPerDraw<NC>::G;
-
-  for (int s = 0; s < a.nsweeps; ++s) {
-    const int64_t k = a.k_begin + s;
-    const uint64_t kk = (uint64_t)(a.philox_base + k);
-    for (int c = 0; c < G.ncol; ++c) {
-      const int b = G.col_ptr[c], cnt = G.col_ptr[c + 1] - b;
-      const int ngrp = (cnt + GD - 1) / GD;
-      if (perm) {
-        for (int q = lt; q < ngrp; q += TPR)
-          metro_group_pat<NC, ST>(sp, b, cnt, q, c, s_pid, s_pat, zmax, s_coup, s_cls, beta, h, kk, key0, key1, dE,
-                                  dM);
-      } else {
-        for (int q = lt; q < ngrp; q += TPR)
-          metro_group<NC, ST>(sp, G.col_site + b, cnt, q, c, G.link, zmax, s_coup, G.cls, s_cls, beta, h, kk, key0,
-                              key1, dE, dM);
-      }
-      __syncthreads();
-    }
-    const bool meas = k >= a.nthermal && ((k - a.nthermal + 1) % a.spf) == 0;
-    const bool therm_end = (k + 1 == a.nthermal);
-    const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
-    if (meas || therm_end || rec || s == a.nsweeps - 1) {
-      if (perm)
-        lds_sweep_tail<NC, ST>(psp, G, B, s_coup, s_cls, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end, rec,
-                               dE, dM, key0, key1);
-      else
-        lds_sweep_tail<NC, ST>(sp, G, B, s_coup, s_cls, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end, rec,
-                               dE, dM, key0, key1);
-    }
+  if (perm) {
+    FrameCtx C{&G, s_coup, s_clsl, s_red, s_frm + g * FRM, g, lt, TPR};
+    lds_sweeps<NC, ST, 0>(G, B, a, sp, psp, s_coup, s_clsl, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
+                          beta, h, key0, key1);
+  } else if (!wide) {
+    FrameCtx C{&G, s_coup, s_clsl, s_red, s_frm + g * FRM, g, lt, TPR};
+    lds_sweeps<NC, ST, 1>(G, B, a, sp, psp, s_coup, s_clsl, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
+                          beta, h, key0, key1);
+  } else {
+    FrameCtx C{&G, G.coup, G.clsdat, s_red, s_frm + g * FRM, g, lt, TPR};
+    lds_sweeps<NC, ST, 2>(G, B, a, sp, psp, G.coup, G.clsdat, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
+                          beta, h, key0, key1);
   }
   if (perm) {
     for (int c = 0; c < NC; ++c)
@@ -840,9 +904,10 @@ __global__ __launch_bounds__(256) void k_pass(mcg_batch B, const int32_t* __rest
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q < ngrp) {
       const int row = rp.row0 + B.rung[rep];
-      const Spins<ST> sp{(ST*)B.spins + rp.spin_off, G.n};
-      metro_group<NC, ST>(sp, G.col_site + b, cnt, q, colour, G.link, G.zmax, G.coup, G.cls, G.clsdat,
-                          B.row_beta[row], B.row_h[row], (uint64_t)sweep, B.seed_lo, B.seed_hi ^ rp.serial, dE, dM);
+      const GSpins<ST> sp{gas((ST*)B.spins + rp.spin_off), G.n};
+      metro_group<NC>(sp, gas(G.col_site) + b, cnt, q, colour, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, G.zmax,
+                      gas(G.coup), gas(G.cls), gas(G.clsdat), B.row_beta[row], B.row_h[row], (uint64_t)sweep,
+                      B.seed_lo, B.seed_hi ^ rp.serial, dE, dM);
     }
   }
   // block reduction, one FP64 atomic per value per block
@@ -1010,7 +1075,8 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
           const int t = (int)(wd & 0xFFFFFFu);
           if (t == o) continue;
           if (__hip_atomic_load(stamp + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == cur) continue;
-          const float* J = G.coup + (wd >> 24) * 9;
+          const uint32_t cw = G.lcid ? (uint32_t)G.lcid[(size_t)o * G.zmax + l] : (wd >> 24);
+          const float* J = G.coup + cw * 9;
           float corr;
           if constexpr (NC == 1) {
             corr = (J[0] * sp.get(t, 0)) * so[0] * beta;  // J*S_t*sigma_t * s_o
@@ -1058,8 +1124,9 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
         for (int l = 0; l < G.zmax; ++l) {
           const uint32_t wd = G.link[(size_t)b * G.zmax + l];
           const int t = (int)(wd & 0xFFFFFFu);
-          const float* J = G.coup + (wd >> 24) * 9;
-          if ((wd >> 24) == 0u) continue;
+          const uint32_t cw = G.lcid ? (uint32_t)G.lcid[(size_t)b * G.zmax + l] : (wd >> 24);
+          const float* J = G.coup + cw * 9;
+          if (cw == 0u) continue;
           float stt[3];
           spin_vec<NC>(sp, t, G.cls, G.clsdat, stt);
           const float sdt = -(stt[0] * r0 + stt[1] * r1 + stt[2] * r2);
@@ -1171,8 +1238,9 @@ int check_batch(const mcg_batch* B) {
   for (int gi = 0; gi < B->ngraph; ++gi) {
     const mcg_graph& G = B->graphs_host[gi];
     if (G.n <= 0 || G.n > (1 << 24)) return fail("graph %d: n=%d outside 1..2^24", gi, G.n);
-    if (G.ncoup < 1 || G.ncoup > 256) return fail("graph %d: ncoup=%d outside 1..256", gi, G.ncoup);
-    if (G.ncls < 1 || G.ncls > 256) return fail("graph %d: ncls=%d outside 1..256", gi, G.ncls);
+    if (G.ncoup < 1 || (G.ncoup > 256 && !G.lcid))
+      return fail("graph %d: ncoup=%d needs per-link coupling indices (lcid) above 256", gi, G.ncoup);
+    if (G.ncls < 1 || G.ncls > G.n) return fail("graph %d: ncls=%d outside 1..n", gi, G.ncls);
     if (G.ncol < 1) return fail("graph %d: no colours", gi);
     if ((B->meas & MCG_MEAS_GROUPS) && G.ngroup > 16) return fail("graph %d: more than 16 orbital groups", gi);
   }
@@ -1277,7 +1345,9 @@ int plan_lds(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool force_stream
       for (int R = nwaves; R >= 1; --R) {
         if (nwaves % R) continue;
         if (P->reps_per_block > 0 && R > P->reps_per_block) continue;
-        if (lds_fixed_bytes(G.ncoup, G.ncls, G.npat, G.zmax, G.n, nwaves, R) + R * per <= X.lds_max) {
+        const bool wide = G.lcid != nullptr || G.ncls > 256;
+        if (lds_fixed_bytes(wide ? 0 : G.ncoup, wide ? 0 : G.ncls, G.ncol, G.npat, G.zmax, G.n, nwaves, R) + R * per <=
+            X.lds_max) {
           Rmax = R;
           break;
         }
@@ -1314,7 +1384,9 @@ size_t lds_bytes_for(const mcg_batch* B, const RunCtx& X) {
   const int nwaves = X.block / 64;
   for (const WorkItem& w : X.hitems) {
     const mcg_graph& G = B->graphs_host[w.graph];
-    const size_t b = lds_fixed_bytes(G.ncoup, G.ncls, G.npat, G.zmax, G.n, nwaves, w.nrep) +
+    const bool wide = G.lcid != nullptr || G.ncls > 256;
+    const size_t b = lds_fixed_bytes(wide ? 0 : G.ncoup, wide ? 0 : G.ncls, G.ncol, G.npat, G.zmax, G.n, nwaves,
+                                     w.nrep) +
                      w.nrep * lds_replica_bytes(NC, G.n, sizeof(ST));
     mx = std::max(mx, b);
   }

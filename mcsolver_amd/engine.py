@@ -5,7 +5,7 @@ the ROCm device), and drives libmcgpu through the C ABI.
 Device layout (HBM), per lattice ("graph"), shared by all its replicas:
   link[n*zmax]   uint32  neighbour | coupling-id << 24   (4 B / link)
   coup[ncoup*9]  fp32    distinct 3x3 tensors, row 0 = zero (<= 256)
-  cls[n] uint8 + clsdat[ncls*4] fp32: (S, Dx, Dy, Dz) per site class
+  cls[n] int32 + clsdat[ncls*4] fp32: (S, Dx, Dy, Dz) per site class
   col_ptr/col_site: sites grouped by colour of a proper graph colouring
 Per replica: spins as structure-of-arrays, component c of replica r at
 spin_off + c*n (fp32, fp16 or int8 for Ising).
@@ -59,6 +59,7 @@ class GraphTables:
     pat: Optional[np.ndarray] = None
     pattab: Optional[np.ndarray] = None
     npat: int = 0
+    lcid: Optional[np.ndarray] = None
     dev: Optional[dict] = None
     struct: Optional[L.Graph] = None
 
@@ -82,7 +83,7 @@ def compile_graph(nc: int, nbr: np.ndarray, J: np.ndarray, S: np.ndarray, D: np.
     n, zmax = nbr.shape
     if n > (1 << 24):
         raise ValueError("more than 2^24 sites per replica")
-    J = np.asarray(J, np.float64).reshape(n, zmax, -1)
+    J = np.asarray(J, np.float64).reshape(n, zmax, -1) if zmax else np.zeros((n, 0, 9))
     S = np.asarray(S, np.float64).reshape(n)
     D = np.asarray(D, np.float64).reshape(n, 3)
     valid = nbr >= 0
@@ -116,15 +117,18 @@ def compile_graph(nc: int, nbr: np.ndarray, J: np.ndarray, S: np.ndarray, D: np.
     remap = np.empty_like(perm)
     remap[perm] = np.arange(len(perm))
     cid = remap[inv[1:]].reshape(n, zmax)
-    if len(uniq) > 256:
-        raise ValueError("more than 256 distinct coupling tensors (%d)" % len(uniq))
-    link = (tgt.astype(np.uint32) | (cid.astype(np.uint32) << 24)).astype(np.uint32)
+    wide = len(uniq) > 256
+    if wide:
+        # per-link coupling indices; link words carry the neighbour only
+        lcid = cid.astype(np.int32).reshape(-1)
+        link = tgt.astype(np.uint32)
+    else:
+        lcid = None
+        link = (tgt.astype(np.uint32) | (cid.astype(np.uint32) << 24)).astype(np.uint32)
 
     cd = np.concatenate([S[:, None], D], 1).astype(np.float32)
     cu, cinv = np.unique(cd, axis=0, return_inverse=True)
-    if len(cu) > 256:
-        raise ValueError("more than 256 distinct site classes")
-    cls = cinv.reshape(-1).astype(np.uint8)
+    cls = cinv.reshape(-1).astype(np.int32)
 
     if colour is None:
         colour = colour_graph(np.where(valid, nbr, -1).astype(np.int32))
@@ -154,7 +158,11 @@ def compile_graph(nc: int, nbr: np.ndarray, J: np.ndarray, S: np.ndarray, D: np.
         ok &= np.where(rl >= 0, rrank[np.clip(rl, 0, n - 1)] >= 0, False)
         rt = np.where(ok, rl, rorb[:, None])
         rc = np.where(ok, rc, 0)
-        rlk = (rt.astype(np.uint32) | (rc.astype(np.uint32) << 24)).astype(np.uint32)
+        if wide:  # the kernel reads the coupling of link l from lcid
+            rt = np.where(ok, rt, -1)
+            rlk = np.where(rt >= 0, rt, 0).astype(np.uint32) | np.where(ok, 0, 0xFF000000).astype(np.uint32)
+        else:
+            rlk = (rt.astype(np.uint32) | (rc.astype(np.uint32) << 24)).astype(np.uint32)
     else:
         rlk = np.zeros((nr, zmax), np.uint32)
     for name, arr in (("corrOrbPair", pairs), ("localCircuits", circ), ("rOrb", rorb), ("rOrbCluster", rclu)):
@@ -177,7 +185,7 @@ def compile_graph(nc: int, nbr: np.ndarray, J: np.ndarray, S: np.ndarray, D: np.
     delta = pos[tgt[site_of]].astype(np.int64) - np.arange(n)[:, None]
     keyrows = np.concatenate([delta, cid[site_of].astype(np.int64), cls[site_of].astype(np.int64)[:, None]], 1)
     prow, pinv = np.unique(keyrows, axis=0, return_inverse=True)
-    if len(prow) <= 256 and n < (1 << 23):
+    if len(prow) <= 256 and n < (1 << 23) and not wide:
         npat = len(prow)
         words = (prow[:, :zmax] & 0xFFFFFF) | (prow[:, zmax:2 * zmax] << 24)
         pattab = np.concatenate([words, prow[:, 2 * zmax:]], 1).astype(np.uint32).view(np.int32).reshape(-1)
@@ -186,7 +194,7 @@ def compile_graph(nc: int, nbr: np.ndarray, J: np.ndarray, S: np.ndarray, D: np.
         npat, pattab, pat = 0, np.zeros(1, np.int32), np.zeros(n, np.uint8)
     return GraphTables(n, zmax, link.reshape(-1), uniq.astype(np.float32).reshape(-1), cls, cu.reshape(-1),
                        col_ptr, order, pairs, circ, rorb, rclu, rlk.reshape(-1), garr, gmax, rrank, colour,
-                       pos=pos, pat=pat, pattab=pattab, npat=npat)
+                       pos=pos, pat=pat, pattab=pattab, npat=npat, lcid=lcid)
 
 
 def _dev(a: np.ndarray, device) -> torch.Tensor:
@@ -204,6 +212,8 @@ def upload_graph(g: GraphTables, device) -> None:
     d = {k: _dev(getattr(g, k), device) for k in ("link", "coup", "cls", "clsdat", "col_ptr", "col_site", "pairs",
                                                   "circ", "rorb", "rclu", "rlink", "groups", "rrank", "pos", "pat",
                                                   "pattab")}
+    if g.lcid is not None:
+        d["lcid"] = _dev(g.lcid, device)
     g.dev = d
     s = L.Graph()
     s.n, s.zmax, s.ncol = g.n, g.zmax, g.ncol

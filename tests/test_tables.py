@@ -89,3 +89,19 @@ def test_random_graph_has_no_patterns():
 def test_bad_indices_rejected():
     with pytest.raises(ValueError):
         compile_graph(3, np.array([[1], [5]]), np.zeros((2, 1, 9)), np.ones(2), np.zeros((2, 3)))
+
+
+def test_wide_coupling_tables():
+    """More than 256 distinct tensors: per-link coupling indices."""
+    rng = np.random.default_rng(3)
+    n, z = 200, 4
+    nbr = rng.integers(0, n, (n, z))
+    J = rng.normal(size=(n, z, 9))
+    g = compile_graph(3, nbr, J, np.ones(n), np.zeros((n, 3)))
+    assert g.lcid is not None and g.npat == 0
+    coup = g.coup.reshape(-1, 9)
+    cid = g.lcid.reshape(n, z)
+    assert np.array_equal(g.link.reshape(n, z).astype(np.int64), nbr)
+    for i in range(n):
+        for l in range(z):
+            assert np.array_equal(coup[cid[i, l]], J[i, l, ROWMAJOR].astype(np.float32))
