@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=20)
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -91,6 +93,12 @@ def main():
     from mcsolver_amd import workloads as W
 
     torch.cuda.set_device(0)
+    if a.copy_calib_mib > 0:
+        src = torch.empty(a.copy_calib_mib << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     K, Wm = a.steps, a.warmup
     wl = W.build("c2", nframes=K + Wm, nbins=16)
     b = wl.batch

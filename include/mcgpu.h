@@ -70,7 +70,8 @@ typedef struct mcg_graph {
   const uint32_t *link;    /* [n*zmax] */
   const float *coup;       /* [ncoup*9] row-major 3x3 (E = s_i^T J s_j), unscaled */
   const int32_t *cls;      /* [n] site class */
-  const float *clsdat;     /* [ncls*4] : S (signed spin length), Dx, Dy, Dz */
+  const float *clsdat;     /* [ncls*8] : S (signed spin length), Dx, Dy, Dz, 1/S^2
+                              (FP32, correctly rounded), 0, 0, 0 */
   const int32_t *col_ptr;  /* [ncol+1] */
   const int32_t *col_site; /* [n] sites grouped by colour */
   const int32_t *pairs;    /* [nlat*2] correlation pairs */

@@ -84,17 +84,24 @@ __host__ __device__ __forceinline__ void sincos_turn(uint32_t v, float* c, float
   pc = fmaf(pc, a2, 4.16666666666666667e-2f);
   pc = fmaf(pc, a2, -0.5f);
   const float cs = fmaf(pc, a2, 1.0f);
-  // even octant 2m: phi = m*pi/2 + a ; odd octant: phi = ((o+1)/2)*pi/2 - a
-  switch (o) {
-    case 0: *c = cs; *s = sn; break;
-    case 1: *c = sn; *s = cs; break;
-    case 2: *c = -sn; *s = cs; break;
-    case 3: *c = -cs; *s = sn; break;
-    case 4: *c = -cs; *s = -sn; break;
-    case 5: *c = -sn; *s = -cs; break;
-    case 6: *c = sn; *s = -cs; break;
-    default: *c = cs; *s = -sn; break;
-  }
+  // even octant 2m: phi = m*pi/2 + a ; odd octant: phi = ((o+1)/2)*pi/2 - a.
+  // Octants 1,2,5,6 swap (cos, sin); cos < 0 in 2..5, sin < 0 in 4..7.
+  const bool swp = ((o + 1u) & 2u) != 0u;
+  const float x = swp ? sn : cs;
+  const float y = swp ? cs : sn;
+  const uint32_t cneg = ((o + 2u) & 4u) << 29;
+  const uint32_t sneg = (o & 4u) << 29;
+  *c = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x) ^ cneg);
+  *s = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, y) ^ sneg);
+}
+
+// Renormalisation factor to bring a vector of squared length nn back to
+// length |S|, given inv_s2 = 1/S^2 (FP32, correctly rounded): one Newton
+// step for 1/sqrt(nn/S^2) from 1, f = 1.5 - 0.5 * nn/S^2.  The reflection
+// move keeps nn within ~1e-7 (FP32) or ~1e-3 (FP16 storage) of S^2, so the
+// renormalised length is exact to O(delta^2).
+__host__ __device__ __forceinline__ float renorm_factor(float nn, float inv_s2) {
+  return fmaf(-0.5f, nn * inv_s2, 1.5f);
 }
 
 }  // namespace mcg

@@ -113,16 +113,6 @@ struct Spins {
 template <typename ST>
 using GSpins = Spins<ST, GAS ST*>;
 
-// colour-permuted replica (LDS): site i lives at position pos[i]
-template <typename ST>
-struct PSpins {
-  ST* p;
-  int n;
-  const int32_t* pos;
-  __device__ __forceinline__ float get(int i, int c) const { return SIO<ST>::ld(p + (size_t)c * n + pos[i]); }
-  __device__ __forceinline__ void set(int i, int c, float v) const { SIO<ST>::st(p + (size_t)c * n + pos[i], v); }
-};
-
 // neighbour accessors: the global link table (site order) or a pattern row
 // (colour-permuted positions, LDS)
 struct GlobalLinks {
@@ -154,8 +144,8 @@ static constexpr double REF_PI = 3.1415926535;  // heisenbergLib.c:6
 template <int NC, typename SV, typename LK, typename CP>
 __device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, int zmax, CP coup, int ci, CP clsdat,
                                            float beta, float h, uint32_t w_a, uint32_t w_b, uint32_t w_c,
-                                           double& dEacc, double* dMacc) {
-  const float S = clsdat[ci * 4 + 0];
+                                           float& dEacc, float* dMacc) {
+  const float S = clsdat[ci * 8 + 0];
   if constexpr (NC == 1) {
     float f = 0.0f;
     for (int l = 0; l < zmax; ++l) f = fmaf(coup[lk.cid(l) * 9], sp.get(lk.nbr(l), 0), f);
@@ -166,11 +156,11 @@ __device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, in
     const float db = dE * beta;
     if (db <= 0.0f || u24(w_c) < exp_neg(-db)) {
       sp.set(i, 0, -sg);
-      dEacc += (double)dE;
-      dMacc[0] += (double)(-2.0f * s);
+      dEacc += dE;
+      dMacc[0] += -2.0f * s;
     }
   } else {
-    const float Dx = clsdat[ci * 4 + 1], Dy = clsdat[ci * 4 + 2], Dz = clsdat[ci * 4 + 3];
+    const float Dx = clsdat[ci * 8 + 1], Dy = clsdat[ci * 8 + 2], Dz = clsdat[ci * 8 + 3];
     const float s0 = sp.get(i, 0), s1 = sp.get(i, 1);
     const float s2 = NC == 3 ? sp.get(i, 2) : 0.0f;
     float H0 = 0.0f, H1 = 0.0f, H2 = 0.0f;
@@ -224,17 +214,17 @@ __device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, in
       float nn = n0 * n0;
       nn = fmaf(n1, n1, nn);
       if constexpr (NC == 3) nn = fmaf(n2, n2, nn);
-      const float f = fabsf(S) / sqrtf(nn);
+      const float f = renorm_factor(nn, clsdat[ci * 8 + 4]);
       n0 = n0 * f;
       n1 = n1 * f;
       n2 = n2 * f;
       sp.set(i, 0, n0);
       sp.set(i, 1, n1);
       if constexpr (NC == 3) sp.set(i, 2, n2);
-      dEacc += (double)dE;
-      dMacc[0] += (double)(sp.get(i, 0) - s0);
-      dMacc[1] += (double)(sp.get(i, 1) - s1);
-      if constexpr (NC == 3) dMacc[2] += (double)(sp.get(i, 2) - s2);
+      dEacc += dE;
+      dMacc[0] += sp.get(i, 0) - s0;
+      dMacc[1] += sp.get(i, 1) - s1;
+      if constexpr (NC == 3) dMacc[2] += sp.get(i, 2) - s2;
     }
   }
 }
@@ -249,12 +239,12 @@ struct SitesPerDraw {
 };
 
 template <int NC, typename SV, typename CP>
-__device__ __forceinline__ void metro_group(const SV& sp, const GAS int32_t* __restrict__ cs, int cnt, int q,
-                                            int colour, const GAS uint32_t* __restrict__ link,
+__device__ __forceinline__ void metro_group(const SV& sp, int b, int cnt, int q, int colour,
+                                            const GAS uint32_t* __restrict__ link,
                                             const GAS int32_t* __restrict__ lcid, int zmax, CP coup,
                                             const GAS int32_t* __restrict__ cls, CP clsdat, float beta, float h,
-                                            uint64_t sweep, uint32_t key0, uint32_t key1, double& dEacc,
-                                            double* dMacc) {
+                                            uint64_t sweep, uint32_t key0, uint32_t key1, float& dEacc,
+                                            float* dMacc) {
   constexpr int G = SitesPerDraw<NC>::G;
   const u32x4 x = philox4x32_10(
       u32x4{(uint32_t)q, (uint32_t)sweep, (uint32_t)(sweep >> 32), TAG_METRO | ((uint32_t)colour << 8)}, key0, key1);
@@ -263,7 +253,7 @@ __device__ __forceinline__ void metro_group(const SV& sp, const GAS int32_t* __r
   for (int g = 0; g < G; ++g) {
     const int p = q * G + g;
     if (p < cnt) {
-      const int i = cs[p];
+      const int i = b + p;
       const GlobalLinks lk{link + (size_t)i * zmax, lcid ? lcid + (size_t)i * zmax : nullptr};
       const uint32_t wa = NC == 3 ? w[0] : 0u;
       const uint32_t wb = NC == 3 ? w[1] : NC == 2 ? w[2 * g] : 0u;
@@ -279,8 +269,8 @@ template <int NC, typename ST>
 __device__ __forceinline__ void metro_group_pat(const Spins<ST>& sp, int b, int cnt, int q, int colour,
                                                 const uint8_t* pat, const int32_t* pattab, int zmax,
                                                 const float* coup, const float* clsdat, float beta, float h,
-                                                uint64_t sweep, uint32_t key0, uint32_t key1, double& dEacc,
-                                                double* dMacc) {
+                                                uint64_t sweep, uint32_t key0, uint32_t key1, float& dEacc,
+                                                float* dMacc) {
   constexpr int G = SitesPerDraw<NC>::G;
   const u32x4 x = philox4x32_10(
       u32x4{(uint32_t)q, (uint32_t)sweep, (uint32_t)(sweep >> 32), TAG_METRO | ((uint32_t)colour << 8)}, key0, key1);
@@ -306,7 +296,7 @@ template <int NC, typename SV>
 __device__ __forceinline__ void spin_vec(const SV& sp, int i, const int32_t* __restrict__ cls, const float* clsdat,
                                          float* v) {
   if constexpr (NC == 1) {
-    v[0] = sp.get(i, 0) * clsdat[cls[i] * 4];
+    v[0] = sp.get(i, 0) * clsdat[cls[i] * 8];
     v[1] = v[2] = 0.0f;
   } else {
     v[0] = sp.get(i, 0);
@@ -325,7 +315,7 @@ __device__ __forceinline__ float site_energy(const SV& sp, int i, const uint32_t
   if constexpr (NC == 1) {
     float f = 0.0f;
     for (int l = 0; l < zmax; ++l) f = fmaf(coup[lk.cid(l) * 9], sp.get(lk.nbr(l), 0), f);
-    const float s = sp.get(i, 0) * clsdat[ci * 4];
+    const float s = sp.get(i, 0) * clsdat[ci * 8];
     return fmaf(0.5f * s, f, -h * s);
   } else {
     const float s0 = sp.get(i, 0), s1 = sp.get(i, 1), s2 = NC == 3 ? sp.get(i, 2) : 0.0f;
@@ -342,9 +332,9 @@ __device__ __forceinline__ float site_energy(const SV& sp, int i, const uint32_t
     e = fmaf(s1, H1, e);
     e = fmaf(s2, H2, e);
     e = 0.5f * e;
-    e = fmaf(clsdat[ci * 4 + 1] * s0, s0, e);
-    e = fmaf(clsdat[ci * 4 + 2] * s1, s1, e);
-    if constexpr (NC == 3) e = fmaf(clsdat[ci * 4 + 3] * s2, s2, e);
+    e = fmaf(clsdat[ci * 8 + 1] * s0, s0, e);
+    e = fmaf(clsdat[ci * 8 + 2] * s1, s1, e);
+    if constexpr (NC == 3) e = fmaf(clsdat[ci * 8 + 3] * s2, s2, e);
     e = fmaf(-h, NC == 3 ? s2 : s0, e);
     return e;
   }
@@ -403,7 +393,7 @@ __device__ __forceinline__ void majority(const SV& sp, const mcg_graph& G, const
     a1 += t[1];
     a2 += t[2];
   }
-  const float S = clsdat[G.cls[site] * 4];
+  const float S = clsdat[G.cls[site] * 8];
   if constexpr (NC == 1) {
     float sg;
     if (a0 > 0.0f)
@@ -467,7 +457,7 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
     }
     if (lt == 0) {
       float t[3];
-      spin_vec<NC>(sp, G.n - 1, G.cls, C.clsdat, t);
+      spin_vec<NC>(sp, G.pos[G.n - 1], G.cls, C.clsdat, t);
       v[7] = t[0];
     }
     group_sum8(v, C.s_red, C.g, TPR);
@@ -483,8 +473,8 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
       spin_vec<NC>(sp, ia, G.cls, C.clsdat, a);
       spin_vec<NC>(sp, ib, G.cls, C.clsdat, b);
       spin_vec<NC>(sp, ic, G.cls, C.clsdat, c);
-      v[0] += (double)signed_area(a, b, c, fabsf(C.clsdat[G.cls[ia] * 4]), fabsf(C.clsdat[G.cls[ib] * 4]),
-                                  fabsf(C.clsdat[G.cls[ic] * 4]));
+      v[0] += (double)signed_area(a, b, c, fabsf(C.clsdat[G.cls[ia] * 8]), fabsf(C.clsdat[G.cls[ib] * 8]),
+                                  fabsf(C.clsdat[G.cls[ic] * 8]));
     }
     group_sum8(v, C.s_red, C.g, TPR);
     if (lt == 0) F[8] = v[0];
@@ -546,8 +536,8 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
       if constexpr (NC == 1)
         eo = -h * s[0];
       else
-        eo = C.clsdat[ci * 4 + 1] * s[0] * s[0] + C.clsdat[ci * 4 + 2] * s[1] * s[1] +
-             C.clsdat[ci * 4 + 3] * s[2] * s[2] - h * (NC == 3 ? s[2] : s[0]);
+        eo = C.clsdat[ci * 8 + 1] * s[0] * s[0] + C.clsdat[ci * 8 + 2] * s[1] * s[1] +
+             C.clsdat[ci * 8 + 3] * s[2] * s[2] - h * (NC == 3 ? s[2] : s[0]);
       w[4] += (double)eo;
     }
     group_sum8(v, C.s_red, C.g, TPR);
@@ -710,9 +700,41 @@ static constexpr int FRM = 96;  // frame scratch doubles per group
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
+// copy one replica between HBM (16-byte aligned) and LDS in 16-byte vectors,
+// four in flight per thread
+typedef unsigned int v4u32 __attribute__((ext_vector_type(4)));
+
+template <typename ST, bool IN>
+__device__ __forceinline__ void stage(ST* lds, ST* g, int count, int lt, int TPR) {
+  const int nbytes = count * (int)sizeof(ST);
+  const int nvec = nbytes >> 4;
+  v4u32* lv = (v4u32*)lds;
+  GAS v4u32* gv = (GAS v4u32*)gas(g);
+  int t = lt;
+  for (; t + 3 * TPR < nvec; t += 4 * TPR) {
+    if constexpr (IN) {
+      const v4u32 a0 = gv[t], a1 = gv[t + TPR], a2 = gv[t + 2 * TPR], a3 = gv[t + 3 * TPR];
+      lv[t] = a0; lv[t + TPR] = a1; lv[t + 2 * TPR] = a2; lv[t + 3 * TPR] = a3;
+    } else {
+      const v4u32 a0 = lv[t], a1 = lv[t + TPR], a2 = lv[t + 2 * TPR], a3 = lv[t + 3 * TPR];
+      gv[t] = a0; gv[t + TPR] = a1; gv[t + 2 * TPR] = a2; gv[t + 3 * TPR] = a3;
+    }
+  }
+  for (; t < nvec; t += TPR) {
+    if constexpr (IN) lv[t] = gv[t];
+    else gv[t] = lv[t];
+  }
+  unsigned char* lb = (unsigned char*)lds;
+  GAS unsigned char* gb = (GAS unsigned char*)gas(g);
+  for (int e = (nvec << 4) + lt; e < nbytes; e += TPR) {
+    if constexpr (IN) lb[e] = gb[e];
+    else gb[e] = lb[e];
+  }
+}
+
 __host__ __device__ inline size_t lds_fixed_bytes(int ncoup, int ncls, int ncol, int npat, int zmax, int n,
                                                   int nwaves, int R) {
-  size_t b = align16((size_t)(ncoup * 9 + ncls * 4) * 4) + align16((size_t)(ncol + 1) * 4);
+  size_t b = align16((size_t)(ncoup * 9 + ncls * 8) * 4) + align16((size_t)(ncol + 1) * 4);
   if (npat > 0) b += align16((size_t)npat * (zmax + 1) * 4) + align16((size_t)n);
   return b + align16((size_t)nwaves * RK * 8) + align16((size_t)R * (MCG_NSTATE + FRM) * 8);
 }
@@ -762,7 +784,7 @@ __device__ __forceinline__ void lds_sweep_tail(const SV& sv, const mcg_graph& G,
 // has one address space.
 template <int NC, typename ST, int MODE>
 __device__ __forceinline__ void lds_sweeps(const mcg_graph& G, const mcg_batch& B, const SweepArgs& a,
-                                           const Spins<ST>& sp, const PSpins<ST>& psp, const float* coup,
+                                           const Spins<ST>& sp, const float* coup,
                                            const float* clsdat, const int32_t* s_colptr, const uint8_t* s_pid,
                                            const int32_t* s_pat, double* s_red, double* st, FrameCtx& C, int g,
                                            int lt, int TPR, int row, float beta, float h, uint32_t key0,
@@ -773,34 +795,35 @@ __device__ __forceinline__ void lds_sweeps(const mcg_graph& G, const mcg_batch& 
   for (int s = 0; s < a.nsweeps; ++s) {
     const int64_t k = a.k_begin + s;
     const uint64_t kk = (uint64_t)(a.philox_base + k);
+    float fE = 0.0f, fM[3] = {0.0f, 0.0f, 0.0f};  // this sweep's accepted changes
     for (int c = 0; c < G.ncol; ++c) {
       const int b = s_colptr[c], cnt = s_colptr[c + 1] - b;
       const int ngrp = (cnt + GD - 1) / GD;
       if constexpr (MODE == 0) {
         for (int q = lt; q < ngrp; q += TPR)
-          metro_group_pat<NC, ST>(sp, b, cnt, q, c, s_pid, s_pat, zmax, coup, clsdat, beta, h, kk, key0, key1, dE,
-                                  dM);
+          metro_group_pat<NC, ST>(sp, b, cnt, q, c, s_pid, s_pat, zmax, coup, clsdat, beta, h, kk, key0, key1, fE,
+                                  fM);
       } else if constexpr (MODE == 1) {
         for (int q = lt; q < ngrp; q += TPR)
-          metro_group<NC>(sp, gas(G.col_site) + b, cnt, q, c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, coup,
-                          gas(G.cls), clsdat, beta, h, kk, key0, key1, dE, dM);
+          metro_group<NC>(sp, b, cnt, q, c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, coup, gas(G.cls),
+                          clsdat, beta, h, kk, key0, key1, fE, fM);
       } else {
         for (int q = lt; q < ngrp; q += TPR)
-          metro_group<NC>(sp, gas(G.col_site) + b, cnt, q, c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax,
-                          gas(coup), gas(G.cls), gas(clsdat), beta, h, kk, key0, key1, dE, dM);
+          metro_group<NC>(sp, b, cnt, q, c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, gas(coup),
+                          gas(G.cls), gas(clsdat), beta, h, kk, key0, key1, fE, fM);
       }
       __syncthreads();
     }
+    dE += (double)fE;
+    dM[0] += (double)fM[0];
+    dM[1] += (double)fM[1];
+    dM[2] += (double)fM[2];
     const bool meas = k >= a.nthermal && ((k - a.nthermal + 1) % a.spf) == 0;
     const bool therm_end = (k + 1 == a.nthermal);
     const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
     if (meas || therm_end || rec || s == a.nsweeps - 1) {
-      if constexpr (MODE == 0)
-        lds_sweep_tail<NC, ST>(psp, G, B, coup, clsdat, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end, rec,
-                               dE, dM, key0, key1);
-      else
-        lds_sweep_tail<NC, ST>(sp, G, B, coup, clsdat, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end, rec,
-                               dE, dM, key0, key1);
+      lds_sweep_tail<NC, ST>(sp, G, B, coup, clsdat, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end, rec, dE,
+                             dM, key0, key1);
     }
   }
 }
@@ -825,7 +848,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   size_t off = 0;
   float* s_coup = (float*)smem;
   float* s_clsl = s_coup + ncl * 9;
-  off += align16((size_t)(ncl * 9 + ncs * 4) * 4);
+  off += align16((size_t)(ncl * 9 + ncs * 8) * 4);
   int32_t* s_colptr = (int32_t*)(smem + off);
   off += align16((size_t)(G.ncol + 1) * 4);
   int32_t* s_pat = (int32_t*)(smem + off);
@@ -840,7 +863,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   ST* s_spin = (ST*)(smem + off);
 
   for (int t = threadIdx.x; t < ncl * 9; t += blockDim.x) s_coup[t] = G.coup[t];
-  for (int t = threadIdx.x; t < ncs * 4; t += blockDim.x) s_clsl[t] = G.clsdat[t];
+  for (int t = threadIdx.x; t < ncs * 8; t += blockDim.x) s_clsl[t] = G.clsdat[t];
   for (int t = threadIdx.x; t <= G.ncol; t += blockDim.x) s_colptr[t] = G.col_ptr[t];
   if (perm) {
     for (int t = threadIdx.x; t < G.npat * (zmax + 1); t += blockDim.x) s_pat[t] = G.pattab[t];
@@ -851,12 +874,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   const mcg_replica rp = B.reps[rep];
   ST* gsp = (ST*)B.spins + rp.spin_off;
   ST* my = s_spin + lds_replica_bytes(NC, n, sizeof(ST)) / sizeof(ST) * g;
-  if (perm) {
-    for (int c = 0; c < NC; ++c)
-      for (int t = lt; t < n; t += TPR) my[(size_t)c * n + t] = gsp[(size_t)c * n + G.col_site[t]];
-  } else {
-    for (int t = lt; t < NC * n; t += TPR) my[t] = gsp[t];
-  }
+  stage<ST, true>(my, gsp, NC * n, lt, TPR);
   double* st = s_st + g * MCG_NSTATE;
   if (lt < MCG_NSTATE) st[lt] = B.state[(size_t)rep * MCG_NSTATE + lt];
   __syncthreads();
@@ -865,26 +883,21 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   const int row = rp.row0 + B.rung[rep];
   const float beta = B.row_beta[row], h = B.row_h[row];
   const Spins<ST> sp{my, n};
-  const PSpins<ST> psp{my, n, G.pos};
   if (perm) {
     FrameCtx C{&G, s_coup, s_clsl, s_red, s_frm + g * FRM, g, lt, TPR};
-    lds_sweeps<NC, ST, 0>(G, B, a, sp, psp, s_coup, s_clsl, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
+    lds_sweeps<NC, ST, 0>(G, B, a, sp, s_coup, s_clsl, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
                           beta, h, key0, key1);
   } else if (!wide) {
     FrameCtx C{&G, s_coup, s_clsl, s_red, s_frm + g * FRM, g, lt, TPR};
-    lds_sweeps<NC, ST, 1>(G, B, a, sp, psp, s_coup, s_clsl, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
+    lds_sweeps<NC, ST, 1>(G, B, a, sp, s_coup, s_clsl, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
                           beta, h, key0, key1);
   } else {
     FrameCtx C{&G, G.coup, G.clsdat, s_red, s_frm + g * FRM, g, lt, TPR};
-    lds_sweeps<NC, ST, 2>(G, B, a, sp, psp, G.coup, G.clsdat, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
+    lds_sweeps<NC, ST, 2>(G, B, a, sp, G.coup, G.clsdat, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
                           beta, h, key0, key1);
   }
-  if (perm) {
-    for (int c = 0; c < NC; ++c)
-      for (int t = lt; t < n; t += TPR) gsp[(size_t)c * n + G.col_site[t]] = my[(size_t)c * n + t];
-  } else {
-    for (int t = lt; t < NC * n; t += TPR) gsp[t] = my[t];
-  }
+  __syncthreads();
+  stage<ST, false>(my, gsp, NC * n, lt, TPR);
   if (lt < MCG_NSTATE) B.state[(size_t)rep * MCG_NSTATE + lt] = st[lt];
 }
 
@@ -897,7 +910,7 @@ __global__ __launch_bounds__(256) void k_pass(mcg_batch B, const int32_t* __rest
   const int rep = rep_list[blockIdx.y];
   const mcg_replica rp = B.reps[rep];
   const mcg_graph G = B.graphs[rp.graph];
-  double dE = 0.0, dM[3] = {0.0, 0.0, 0.0};
+  float dE = 0.0f, dM[3] = {0.0f, 0.0f, 0.0f};
   if (colour < G.ncol) {
     const int b = G.col_ptr[colour], cnt = G.col_ptr[colour + 1] - b;
     const int ngrp = (cnt + SitesPerDraw<NC>::G - 1) / SitesPerDraw<NC>::G;
@@ -905,8 +918,8 @@ __global__ __launch_bounds__(256) void k_pass(mcg_batch B, const int32_t* __rest
     if (q < ngrp) {
       const int row = rp.row0 + B.rung[rep];
       const GSpins<ST> sp{gas((ST*)B.spins + rp.spin_off), G.n};
-      metro_group<NC>(sp, gas(G.col_site) + b, cnt, q, colour, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, G.zmax,
-                      gas(G.coup), gas(G.cls), gas(G.clsdat), B.row_beta[row], B.row_h[row], (uint64_t)sweep,
+      metro_group<NC>(sp, b, cnt, q, colour, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, G.zmax, gas(G.coup),
+                      gas(G.cls), gas(G.clsdat), B.row_beta[row], B.row_h[row], (uint64_t)sweep,
                       B.seed_lo, B.seed_hi ^ rp.serial, dE, dM);
     }
   }
@@ -1145,7 +1158,7 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
         }
         // single-ion term with the half step s + sdn*r (heisenbergLib.c:418)
         const int ci = G.cls[b];
-        const float* Dd = G.clsdat + ci * 4 + 1;
+        const float* Dd = G.clsdat + ci * 8 + 1;
         const float tr[3] = {sdb * r0, sdb * r1, sdb * r2};
         float o = 0.0f;
         for (int k = 0; k < NC; ++k) {
@@ -1178,7 +1191,7 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
           const float a2 = -2.0f * (sb[0] * r0 + sb[1] * r1 + sb[2] * r2);
           float n0 = sb[0] + a2 * r0, n1 = sb[1] + a2 * r1, n2 = sb[2] + a2 * r2;
           float nn = n0 * n0 + n1 * n1 + n2 * n2;
-          const float fct = fabsf(G.clsdat[G.cls[b] * 4]) / sqrtf(nn);
+          const float fct = fabsf(G.clsdat[G.cls[b] * 8]) / sqrtf(nn);
           sp.set(b, 0, n0 * fct);
           sp.set(b, 1, n1 * fct);
           if constexpr (NC == 3) sp.set(b, 2, n2 * fct);

@@ -5,7 +5,7 @@ the ROCm device), and drives libmcgpu through the C ABI.
 Device layout (HBM), per lattice ("graph"), shared by all its replicas:
   link[n*zmax]   uint32  neighbour | coupling-id << 24   (4 B / link)
   coup[ncoup*9]  fp32    distinct 3x3 tensors, row 0 = zero (<= 256)
-  cls[n] int32 + clsdat[ncls*4] fp32: (S, Dx, Dy, Dz) per site class
+  cls[n] int32 + clsdat[ncls*8] fp32: (S, Dx, Dy, Dz, 1/S^2, 0, 0, 0) per site class
   col_ptr/col_site: sites grouped by colour of a proper graph colouring
 Per replica: spins as structure-of-arrays, component c of replica r at
 spin_off + c*n (fp32, fp16 or int8 for Ising).
@@ -129,6 +129,11 @@ def compile_graph(nc: int, nbr: np.ndarray, J: np.ndarray, S: np.ndarray, D: np.
     cd = np.concatenate([S[:, None], D], 1).astype(np.float32)
     cu, cinv = np.unique(cd, axis=0, return_inverse=True)
     cls = cinv.reshape(-1).astype(np.int32)
+    # per class: S, Dx, Dy, Dz, 1/S^2 in FP32 (correctly rounded), padding
+    s32 = cu[:, 0].astype(np.float32)
+    with np.errstate(divide="ignore"):
+        inv_s2 = np.where(s32 != 0, np.float32(1.0) / (s32 * s32), np.float32(0.0)).astype(np.float32)
+    cu = np.concatenate([cu, inv_s2[:, None], np.zeros((len(cu), 3), np.float32)], 1).astype(np.float32)
 
     if colour is None:
         colour = colour_graph(np.where(valid, nbr, -1).astype(np.int32))
@@ -178,12 +183,31 @@ def compile_graph(nc: int, nbr: np.ndarray, J: np.ndarray, S: np.ndarray, D: np.
     garr = -np.ones((max(len(groups), 1), gmax), np.int32)
     for k, g in enumerate(groups):
         garr[k, :len(g)] = g
-    # colour-permuted link patterns for the LDS-resident kernel
+    # ---- position space.  Spins live in HBM and LDS in colour order:
+    # position p holds site order[p]; colour c is the contiguous range
+    # col_ptr[c] .. col_ptr[c+1].  Every table the kernels read is
+    # translated to positions here, so no kernel indexes by site.
     pos = np.empty(n, np.int32)
     pos[order] = np.arange(n, dtype=np.int32)
     site_of = order.astype(np.int64)
+    link = link.reshape(n, zmax)[site_of]
+    link = ((pos[(link & 0xFFFFFF).astype(np.int64)].astype(np.uint32)) | (link & np.uint32(0xFF000000))).astype(
+        np.uint32)
+    if lcid is not None:
+        lcid = lcid.reshape(n, zmax)[site_of].reshape(-1)
+    cls_pos = cls[site_of]
+    pairs = pos[pairs] if len(pairs) else pairs
+    circ = pos[circ] if len(circ) else circ
+    rrank_pos = -np.ones(n, np.int32)
+    if nr:
+        rrank_pos[pos[rorb]] = np.arange(nr, dtype=np.int32)
+        rclu = pos[rclu]
+        rt = (rlk & 0xFFFFFF).astype(np.int64)
+        rlk = (pos[rt].astype(np.uint32) | (rlk & np.uint32(0xFF000000))).astype(np.uint32)
+        rorb = pos[rorb]
+    garr = np.where(garr >= 0, pos[np.clip(garr, 0, n - 1)], -1).astype(np.int32)
     delta = pos[tgt[site_of]].astype(np.int64) - np.arange(n)[:, None]
-    keyrows = np.concatenate([delta, cid[site_of].astype(np.int64), cls[site_of].astype(np.int64)[:, None]], 1)
+    keyrows = np.concatenate([delta, cid[site_of].astype(np.int64), cls_pos.astype(np.int64)[:, None]], 1)
     prow, pinv = np.unique(keyrows, axis=0, return_inverse=True)
     if len(prow) <= 256 and n < (1 << 23) and not wide:
         npat = len(prow)
@@ -192,8 +216,8 @@ def compile_graph(nc: int, nbr: np.ndarray, J: np.ndarray, S: np.ndarray, D: np.
         pat = pinv.reshape(-1).astype(np.uint8)
     else:
         npat, pattab, pat = 0, np.zeros(1, np.int32), np.zeros(n, np.uint8)
-    return GraphTables(n, zmax, link.reshape(-1), uniq.astype(np.float32).reshape(-1), cls, cu.reshape(-1),
-                       col_ptr, order, pairs, circ, rorb, rclu, rlk.reshape(-1), garr, gmax, rrank, colour,
+    return GraphTables(n, zmax, link.reshape(-1), uniq.astype(np.float32).reshape(-1), cls_pos, cu.reshape(-1),
+                       col_ptr, order, pairs, circ, rorb, rclu, rlk.reshape(-1), garr, gmax, rrank_pos, colour,
                        pos=pos, pat=pat, pattab=pattab, npat=npat, lcid=lcid)
 
 
@@ -217,7 +241,7 @@ def upload_graph(g: GraphTables, device) -> None:
     g.dev = d
     s = L.Graph()
     s.n, s.zmax, s.ncol = g.n, g.zmax, g.ncol
-    s.ncoup, s.ncls = len(g.coup) // 9, len(g.clsdat) // 4
+    s.ncoup, s.ncls = len(g.coup) // 9, len(g.clsdat) // 8
     s.nlat, s.ncirc, s.nr = len(g.pairs), len(g.circ), len(g.rorb)
     s.nclu = g.rclu.shape[1] if len(g.rorb) else 0
     s.ngroup = 0 if (g.groups < 0).all() else g.groups.shape[0]
@@ -259,19 +283,23 @@ class Batch:
         nrep = len(reps)
         nrow = len(beta)
         self.nrow = nrow
-        # spin pool
+        # spin pool: per replica nc component arrays in colour order
+        # (position p = site col_site[p]), each replica 16-byte aligned
+        esz = {"f32": 4, "f16": 2, "i8": 1}[store]
+        al = 16 // esz
         offs, soffs, tot, stot = [], [], 0, 0
         for r in reps:
             n = graphs[r.graph].n
             offs.append(tot)
             soffs.append(stot)
-            tot += nc * n
+            tot += (nc * n + al - 1) // al * al
             stot += n
         self.spin_off = offs
         pool = np.zeros(max(tot, 1), np.float32)
         for k, (r, sp) in enumerate(zip(reps, init_spins)):
-            n = graphs[r.graph].n
-            a = np.asarray(sp, np.float64).reshape(n, -1)
+            g = graphs[r.graph]
+            n = g.n
+            a = np.asarray(sp, np.float64).reshape(n, -1)[g.col_site]
             if nc == 1:
                 pool[offs[k]:offs[k] + n] = np.where(a[:, 0] >= 0, 1.0, -1.0)
             else:
@@ -381,11 +409,14 @@ class Batch:
         return self.state.cpu().numpy().reshape(-1, L.NSTATE)
 
     def replica_spins(self, k: int) -> np.ndarray:
-        """[n, nc] spins of replica k (Ising: sigma*S is not applied)."""
-        n = self.graphs[self.reps[k].graph].n
+        """[n, nc] spins of replica k in site order (Ising: sigma, S not applied)."""
+        g = self.graphs[self.reps[k].graph]
+        n = g.n
         o = self.spin_off[k]
         a = self.spins[o:o + self.nc * n].float().cpu().numpy().reshape(self.nc, n).T
-        return a
+        out = np.empty_like(a)
+        out[g.col_site] = a
+        return out
 
     def error_flag(self) -> int:
         torch.cuda.synchronize()

@@ -17,9 +17,10 @@
  *     G = 2 (XY: site g takes phi from x_{2g}, accept u(x_{2g+1})),
  *     G = 4 (Ising: site g accepts with u(x_g))
  *   u(x) = (x >> 8) * 2^-24
- *   Wolff cluster c: ctr (0, c_lo, c_hi, 2): seed = x0*n >> 32, direction
- *     from x1 (XY phi; Heisenberg r_z) and x2 (Heisenberg phi), accept x3>>8
- *   Wolff bond (o,l): ctr (o*zmax + l, c_lo, c_hi, 3)
+ *   Wolff cluster c: ctr (0, c_lo, c_hi, 2): seed position = x0*n >> 32,
+ *     direction from x1 (XY phi; Heisenberg r_z) and x2 (Heisenberg phi),
+ *     accept x3>>8.  Positions are indices in colour order.
+ *   Wolff bond (o,l): ctr (pos(o)*zmax + l, c_lo, c_hi, 3)
  *   tempering pair t of ladder L at swap s: ctr (t, s_lo, s_hi, 4),
  *     key (seed_lo, seed_hi ^ (0x80000000 | L))
  */
@@ -234,7 +235,9 @@ static void cb_site(const cb_graph *g, float *sp, long i, float beta, float h, u
     float nn = n0 * n0;
     nn = fmaf(n1, n1, nn);
     if (nc == 3) nn = fmaf(n2, n2, nn);
-    float f = fabsf(S) / sqrtf(nn);
+    /* one Newton step for 1/sqrt(nn/S^2) starting from 1 */
+    float inv_s2 = 1.0f / (S * S);
+    float f = fmaf(-0.5f, nn * inv_s2, 1.5f);
     n0 = n0 * f;
     n1 = n1 * f;
     n2 = n2 * f;
@@ -378,18 +381,29 @@ int mcref_cb_pt(int nr, const float *beta, const double *E_of_rep, int *who, int
  * does not depend on the order the frontier is processed in.
  */
 int mcref_cb_wolff(int nc, int store, long n, int zmax, const int *nbr, const double *J, const double *S,
-                   const double *D, int diag_only, float beta, float h, uint32_t serial, uint32_t seed_lo,
-                   uint32_t seed_hi, long c0, long ncl, float *sp, long *sizes) {
+                   const double *D, int diag_only, const int *colour, int ncol, float beta, float h,
+                   uint32_t serial, uint32_t seed_lo, uint32_t seed_hi, long c0, long ncl, float *sp, long *sizes) {
   cb_graph g;
   cb_build(&g, nc, store, n, zmax, nbr, J, S, D, diag_only);
   char *in = malloc((size_t)n);
   long *q = malloc(sizeof(long) * n);
+  /* the engine indexes sites by their position in colour order: the seed is
+     drawn as a position and bond counters use the position of the site */
+  long *order = malloc(sizeof(long) * n), *pos = malloc(sizeof(long) * n);
+  {
+    long *cnt = calloc((size_t)ncol + 1, sizeof(long));
+    for (long i = 0; i < n; i++) cnt[colour[i] + 1]++;
+    for (int c = 0; c < ncol; c++) cnt[c + 1] += cnt[c];
+    for (long i = 0; i < n; i++) order[cnt[colour[i]]++] = i;
+    for (long p = 0; p < n; p++) pos[order[p]] = p;
+    free(cnt);
+  }
   uint32_t k0 = seed_lo, k1 = seed_hi ^ serial;
   for (long c = 0; c < ncl; c++) {
     uint64_t cl = (uint64_t)(c0 + c);
     memset(in, 0, (size_t)n);
     ref_u4 x = ref_philox(0u, (uint32_t)cl, (uint32_t)(cl >> 32), 2u, k0, k1);
-    long seed = (long)(((uint64_t)x.v[0] * (uint64_t)n) >> 32);
+    long seed = order[(long)(((uint64_t)x.v[0] * (uint64_t)n) >> 32)];
     float r[3] = {0, 0, 0};
     if (nc == 3) {
       r[2] = fmaf(uu(x.v[1]), 2.0f, -1.0f);
@@ -429,7 +443,7 @@ int mcref_cb_wolff(int nc, int store, long n, int zmax, const int *nbr, const do
           corr = 2.0f * sdo * sdt * rJr * beta;
         }
         if (corr < 0.0f) {
-          ref_u4 y = ref_philox((uint32_t)(o * zmax + l), (uint32_t)cl, (uint32_t)(cl >> 32), 3u, k0, k1);
+          ref_u4 y = ref_philox((uint32_t)(pos[o] * zmax + l), (uint32_t)cl, (uint32_t)(cl >> 32), 3u, k0, k1);
           if (1.0f - cexp_neg(corr) > uu(y.v[0])) {
             in[t] = 1;
             q[tail++] = t;
@@ -498,6 +512,8 @@ int mcref_cb_wolff(int nc, int store, long n, int zmax, const int *nbr, const do
   }
   free(in);
   free(q);
+  free(order);
+  free(pos);
   cb_free(&g);
   return 0;
 }

@@ -36,7 +36,7 @@ def lib() -> C.CDLL:
         _l.mcref_cb_sweeps.argtypes = [I, I, L_, I, P, P, P, P, I, P, I, I, P, P, P, U, U, L_, L_, P, P, P]
         _l.mcref_cb_energy.argtypes = [I, L_, I, P, P, P, P, I, F, P, P]
         _l.mcref_cb_pt.argtypes = [I, P, P, P, P, U, U, I, L_, P]
-        _l.mcref_cb_wolff.argtypes = [I, I, L_, I, P, P, P, P, I, F, F, U, U, U, L_, L_, P, P]
+        _l.mcref_cb_wolff.argtypes = [I, I, L_, I, P, P, P, P, I, P, I, F, F, U, U, U, L_, L_, P, P]
         _l.mcref_philox.argtypes = [U, U, U, U, U, U, P]
         _l.mcref_exp_neg.argtypes = [F]
         _l.mcref_exp_neg.restype = F
@@ -159,7 +159,7 @@ def cb_pt(beta, E_of_rep, who, rung, seed, ladder, swap_index, acc):
                       ladder, swap_index, ptr(acc))
 
 
-def cb_wolff(nc, store, nbr, J, S, D, beta, h, serial, seed, c0, ncl, spins1, diag_only=0):
+def cb_wolff(nc, store, nbr, J, S, D, colour, beta, h, serial, seed, c0, ncl, spins1, diag_only=0):
     nbr = _a(nbr, np.int32); n, z = nbr.shape
     J = _a(J, np.float64).reshape(n, z, -1)
     if J.shape[2] < 9:
@@ -167,9 +167,10 @@ def cb_wolff(nc, store, nbr, J, S, D, beta, h, serial, seed, c0, ncl, spins1, di
     J = _a(J, np.float64)
     assert spins1.dtype == np.float32 and spins1.flags.c_contiguous
     sizes = np.zeros(ncl, np.int64)
+    colour = _a(colour, np.int32)
     lib().mcref_cb_wolff(nc, store, n, z, ptr(nbr), ptr(J), ptr(_a(S, np.float64)), ptr(_a(D, np.float64)), diag_only,
-                         float(beta), float(h), serial, seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF, c0, ncl,
-                         ptr(spins1), ptr(sizes))
+                         ptr(colour), int(colour.max()) + 1, float(beta), float(h), serial, seed & 0xFFFFFFFF,
+                         (seed >> 32) & 0xFFFFFFFF, c0, ncl, ptr(spins1), ptr(sizes))
     return sizes
 
 

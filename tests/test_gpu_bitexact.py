@@ -136,7 +136,7 @@ def test_wolff_bitexact(gpu, case):
     nbr, J, S, D = _lattice_tables(Lt, nc)
     for k in range(len(betas)):
         one = np.ascontiguousarray(ref[k])
-        mcref.cb_wolff(nc, 0 if nc > 1 else 2, nbr, J, S, D, betas[k], h, k, SEED, 0, 2 * ncl, one)
+        mcref.cb_wolff(nc, 0 if nc > 1 else 2, nbr, J, S, D, g.colour, betas[k], h, k, SEED, 0, 2 * ncl, one)
         ref[k] = one
     if nc == 1:
         assert np.array_equal(got, ref)

@@ -14,13 +14,19 @@ def _check(nc, L_):
     J = L_.J if nc > 1 else L_.J[:, :, :1]
     g = compile_graph(nc, L_.nbr, J, L_.S, L_.D)
     n, z = L_.nbr.shape
-    link = g.link.reshape(n, z).astype(np.int64)
-    nbr = link & 0xFFFFFF
+    # tables are in colour-order positions: position p is site col_site[p]
+    assert sorted(g.col_site.tolist()) == list(range(n))
+    assert np.array_equal(g.pos[g.col_site], np.arange(n))
+    linkp = g.link.reshape(n, z).astype(np.int64)
+    link = np.empty_like(linkp)
+    link[g.col_site] = linkp
+    nbr = g.col_site[link & 0xFFFFFF]
     cid = link >> 24
     coup = g.coup.reshape(-1, 9)
     valid = L_.nbr >= 0
     assert np.array_equal(nbr[valid], L_.nbr[valid])
-    assert (nbr[~valid] == np.nonzero(~valid)[0]).all() if (~valid).any() else True
+    if (~valid).any():
+        assert (nbr[~valid] == np.nonzero(~valid)[0]).all()
     assert np.all(coup[0] == 0)
     for i in range(n):
         for l in range(z):
@@ -44,10 +50,17 @@ def _check(nc, L_):
                 assert col[i] != col[j]
     assert sorted(g.col_site.tolist()) == list(range(n))
     assert g.col_ptr[-1] == n
-    # class table
-    cd = g.clsdat.reshape(-1, 4)
-    assert np.array_equal(cd[g.cls, 0], L_.S.astype(np.float32))
-    assert np.array_equal(cd[g.cls, 1:], L_.D.astype(np.float32))
+    # class table (cls is per position)
+    cd = g.clsdat.reshape(-1, 8)
+    cls = np.empty_like(g.cls)
+    cls[g.col_site] = g.cls
+    assert np.array_equal(cd[cls, 0], L_.S.astype(np.float32))
+    assert np.array_equal(cd[cls, 1:4], L_.D.astype(np.float32))
+    s32 = L_.S.astype(np.float32)
+    assert np.array_equal(cd[cls, 4], np.float32(1) / (s32 * s32))
+    # colour classes are contiguous position ranges
+    for c in range(g.ncol):
+        assert (col[g.col_site[g.col_ptr[c]:g.col_ptr[c + 1]]] == c).all()
     # link patterns in colour-permuted positions
     if g.npat:
         pt = g.pattab.view(np.uint32).reshape(g.npat, z + 1).astype(np.int64)
@@ -57,8 +70,7 @@ def _check(nc, L_):
             d = ((row[:z] & 0xFFFFFF) ^ 0x800000) - 0x800000
             assert np.array_equal(g.col_site[p + d], nbr[i]), p
             assert np.array_equal(row[:z] >> 24, cid[i])
-            assert row[z] == g.cls[i]
-        assert np.array_equal(g.pos[g.col_site], np.arange(n))
+            assert row[z] == cls[i]
     return g
 
 
@@ -100,8 +112,11 @@ def test_wide_coupling_tables():
     g = compile_graph(3, nbr, J, np.ones(n), np.zeros((n, 3)))
     assert g.lcid is not None and g.npat == 0
     coup = g.coup.reshape(-1, 9)
-    cid = g.lcid.reshape(n, z)
-    assert np.array_equal(g.link.reshape(n, z).astype(np.int64), nbr)
+    cid = np.empty((n, z), np.int64)
+    cid[g.col_site] = g.lcid.reshape(n, z)
+    lk = np.empty((n, z), np.int64)
+    lk[g.col_site] = g.link.reshape(n, z).astype(np.int64)
+    assert np.array_equal(g.col_site[lk], nbr)
     for i in range(n):
         for l in range(z):
             assert np.array_equal(coup[cid[i, l]], J[i, l, ROWMAJOR].astype(np.float32))
