@@ -74,7 +74,17 @@ def _run_frames(b: Batch, nth: int, nsweep: int, spf: int, alg: int, ncl: int, s
 
 def ising_main(algorithm, initSpin, nthermal, nsweep, ninterval, maxNLinking, nlink, linkStrength, linkedOrb,
                corrOrbPair, h, rOrb, rOrbCluster, linkedOrb_rnorm, spinFrame, callback=None,
-               seed: int = DEFAULT_SEED):
+               seed: int = DEFAULT_SEED, per_link_couplings: bool = False):
+    """isinglib.MCMainFunction on the GPU.
+
+    By default link l of site i takes coupling linkStrength[i + l], as the
+    reference reads it (isingLib.c:30).  With equal couplings on every link
+    that is the same as linkStrength[i*maxNLinking + l]; with unequal ones
+    the reference's update rule is not derived from any Hamiltonian and its
+    stationary state depends on the update order, so no coloured schedule
+    reproduces it.  per_link_couplings=True uses linkStrength[i*maxNLinking
+    + l], the Hamiltonian the input describes.
+    """
     S = np.asarray(initSpin, np.float64)
     n = len(S)
     maxl = int(maxNLinking)
@@ -87,6 +97,8 @@ def ising_main(algorithm, initSpin, nthermal, nsweep, ninterval, maxNLinking, nl
     nbr = np.where(lmask, nbr, -1)
     # coupling of link l of site i is entry i + l of the flat table (isingLib.c:30)
     idx = np.arange(n)[:, None] + np.arange(maxl)[None, :]
+    if per_link_couplings:
+        idx = np.arange(n)[:, None] * maxl + np.arange(maxl)[None, :]
     Jeff = np.where(lmask, flat[np.clip(idx, 0, flat.size - 1)], 0.0)
     pairs = np.asarray(corrOrbPair, np.int64).reshape(-1, 2)
     nr = len(rOrb)
