@@ -20,6 +20,7 @@
 // (spins, site, sweep index, replica key), with a counter-based Philox
 // stream, so all kernels produce the same trajectory bit for bit.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
@@ -1276,6 +1277,9 @@ struct RunCtx {
 };
 
 // HIP-event bracketing of the sweep-kernel launches (plan->timing)
+// Per-launch kernel timing: a start/stop event pair per timed launch,
+// stamped by the kernel's own dispatch (hipExtLaunchKernelGGL), so the
+// measured span is the kernel's execution and not the queue around it.
 struct LaunchTimer {
   bool on = false;
   std::vector<hipEvent_t> ev;
@@ -1283,12 +1287,14 @@ struct LaunchTimer {
   ~LaunchTimer() {
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
   }
-  int mark(hipStream_t s) {
+  // two fresh events when timing is on, null otherwise
+  int pair(hipEvent_t* a, hipEvent_t* b) {
+    *a = *b = nullptr;
     if (!on) return 0;
-    hipEvent_t e;
-    HIPCHK(hipEventCreate(&e));
-    ev.push_back(e);
-    HIPCHK(hipEventRecord(e, s));
+    HIPCHK(hipEventCreate(a));
+    ev.push_back(*a);
+    HIPCHK(hipEventCreate(b));
+    ev.push_back(*b);
     return 0;
   }
   int finish(double* out) {
@@ -1407,12 +1413,13 @@ size_t lds_bytes_for(const mcg_batch* B, const RunCtx& X) {
 }
 
 template <int NC, typename ST>
-int launch_lds(const mcg_batch* B, RunCtx& X, const SweepArgs& a, hipStream_t s) {
+int launch_lds(const mcg_batch* B, RunCtx& X, const SweepArgs& a, hipStream_t s, hipEvent_t e0 = nullptr,
+               hipEvent_t e1 = nullptr) {
   if (X.hitems.empty()) return 0;
   const size_t bytes = lds_bytes_for<NC, ST>(B, X);
   set_lds_attr<NC, ST>(bytes);
-  hipLaunchKernelGGL((k_lds<NC, ST>), dim3((unsigned)X.hitems.size()), dim3(X.block), bytes, s, *B,
-                     (const WorkItem*)X.items.p, (const int32_t*)X.item_reps.p, a);
+  hipExtLaunchKernelGGL((k_lds<NC, ST>), dim3((unsigned)X.hitems.size()), dim3(X.block), (uint32_t)bytes, s, e0,
+                        e1, 0, *B, (const WorkItem*)X.items.p, (const int32_t*)X.item_reps.p, a);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1430,10 +1437,11 @@ int max_colour_count(const mcg_batch* B, const std::vector<int32_t>& reps, int* 
 
 template <int NC, typename ST>
 int launch_pass(const mcg_batch* B, const int32_t* dreps, int nreps, int maxn, int colour, int64_t sweep,
-                hipStream_t s) {
+                hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   if (nreps == 0) return 0;
   const unsigned chunks = (unsigned)((maxn + 255) / 256);
-  hipLaunchKernelGGL((k_pass<NC, ST>), dim3(chunks, (unsigned)nreps), dim3(256), 0, s, *B, dreps, colour, sweep);
+  hipExtLaunchKernelGGL((k_pass<NC, ST>), dim3(chunks, (unsigned)nreps), dim3(256), 0, s, e0, e1, 0, *B, dreps,
+                        colour, sweep);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1476,11 +1484,13 @@ int stream_sweeps(const mcg_batch* B, const mcg_plan* P, RunCtx& X, int64_t k0, 
     for (int c = 0; c < ncolmax; ++c) {
       if (plain) {
         for (int i = 0; i < nrs; ++i) {
-          if (T.mark(s) || launch_pass<NC, ST>(B, dreps + i, 1, maxn, c, kk, s) || T.mark(s)) return -1;
+          hipEvent_t e0, e1;
+          if (T.pair(&e0, &e1) || launch_pass<NC, ST>(B, dreps + i, 1, maxn, c, kk, s, e0, e1)) return -1;
           T.launches += 1;
         }
       } else {
-        if (T.mark(s) || launch_pass<NC, ST>(B, dreps, nrs, maxn, c, kk, s) || T.mark(s)) return -1;
+        hipEvent_t e0, e1;
+        if (T.pair(&e0, &e1) || launch_pass<NC, ST>(B, dreps, nrs, maxn, c, kk, s, e0, e1)) return -1;
         T.launches += 1;
       }
     }
@@ -1574,7 +1584,8 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
     const int64_t ns = std::min<int64_t>(chunk, total - k);
     SweepArgs a{k, P->sweep0, (int32_t)ns, (int32_t)nth, spf, P->frame0};
     if (!X.hitems.empty()) {
-      if (T.mark(s) || launch_lds<NC, ST>(B, X, a, s) || T.mark(s)) return -1;
+      hipEvent_t e0, e1;
+      if (T.pair(&e0, &e1) || launch_lds<NC, ST>(B, X, a, s, e0, e1)) return -1;
       T.launches += 1;
       T.sweeps += (double)ns;
       T.updates += lds_updates_per_sweep * (double)ns;

@@ -45,10 +45,14 @@ def _plan_counts(n: int, nthermal: int, nsweep: int, ninterval: int, wolff: bool
     return nthermal * spf, spf, 0
 
 
-def _run_frames(b: Batch, nth: int, nsweep: int, spf: int, alg: int, ncl: int, spinFrame: int, nc: int):
-    """Run thermalisation + nsweep frames; collect spin snapshots at frames
-    0, ops, 2*ops, ... (isingLib.c:366-386)."""
-    frames = []
+def _run_frames(b: Batch, nth: int, nsweep: int, spf: int, alg: int, ncl: int, spinFrame: int,
+                replicas=(0,)) -> list:
+    """Run thermalisation + nsweep frames; collect spin snapshots of the given
+    replicas at frames 0, ops, 2*ops, ... with ops = nsweep // spinFrame
+    (isingLib.c:366-386).  Returns frames[k] = list of [n, nc] arrays of
+    replica replicas[k].  Philox counters depend only on the sweep index, so
+    splitting the run at the snapshots leaves the trajectory unchanged."""
+    frames = [[] for _ in replicas]
     if spinFrame <= 0:
         b.run(nth, nsweep, spf, algorithm=alg, clusters_per_sweep=ncl)
         return frames
@@ -67,9 +71,80 @@ def _run_frames(b: Batch, nth: int, nsweep: int, spf: int, alg: int, ncl: int, s
             sweep0 += th + todo * spf
             done += todo
             first = False
-        if m < nsweep and len(frames) < spinFrame:
-            frames.append(b.replica_spins(0).copy())
+        if m < nsweep and len(frames[0]) < spinFrame:
+            for k, r in enumerate(replicas):
+                frames[k].append(b.replica_spins(r).copy())
     return frames
+
+
+def ising_couplings(linkStrength, n: int, maxl: int, nlink, per_link_couplings: bool = False) -> np.ndarray:
+    """[n, maxl] coupling of link l of site i: entry i + l of the flat table,
+    as isingLib.c:30 reads it, or entry i*maxl + l with per_link_couplings.
+    Links beyond nlink[i] get 0."""
+    flat = np.asarray(linkStrength, np.float64).reshape(-1)
+    nl = np.asarray(nlink, np.int64)
+    lmask = np.arange(maxl)[None, :] < nl[:, None]
+    if per_link_couplings:
+        idx = np.arange(n)[:, None] * maxl + np.arange(maxl)[None, :]
+    else:
+        idx = np.arange(n)[:, None] + np.arange(maxl)[None, :]
+    return np.where(lmask, flat[np.clip(idx, 0, flat.size - 1)], 0.0)
+
+
+def ising_observables(A: np.ndarray, nsweep: int, S: np.ndarray, ninterval: int, wolff: bool, nLat: int) -> tuple:
+    """The first ten entries of isinglib's result (isingLib.c:436-446) from
+    one row of engine accumulators: spin_i, spin_j, spin_ij, autoCorr, E, E2,
+    E_rnorm, E2_rnorm, U4, spin_tot.  S = initial spins (all +S)."""
+    ns = float(nsweep)
+    n = len(S)
+    # reference totSpin starts at the sum of the first ninterval spins
+    # (isingLib.c:350) and, under Wolff, misses flips of the last site
+    # (isingLib.c:228)
+    off = S[: min(int(ninterval), n)].sum() - S.sum()
+    tot = A[L.A_TOT] / ns + off
+    if wolff:
+        tot += -A[L.A_LAST] / ns + S[n - 1]
+    U4 = (A[L.A_M2] / ns) * (A[L.A_M2] / ns) / (A[L.A_M4] / ns)
+    return (A[L.A_SI] / ns, A[L.A_SJ] / ns, A[L.A_SIJ] / ns, A[L.A_MM] / ns - (A[L.A_M] / ns) * (A[L.A_M] / ns),
+            A[L.A_E] / ns, A[L.A_E2] / ns, A[L.A_ER] / ns, A[L.A_ER2] / ns, U4, tot / nLat)
+
+
+def on_observables(A: np.ndarray, nsweep: int, nc: int, has_rnorm: bool) -> list:
+    """The first 27 entries of {xy,heisenberg}lib's result
+    (heisenbergLib.c:854-880, xyLib.c:781-805) from one row of engine
+    accumulators."""
+    ns = float(nsweep)
+    three = (lambda base: [A[base] / ns, A[base + 1] / ns, A[base + 2] / ns if nc == 3 else 0.0])
+    Mb = A[L.A_M] / ns
+    auto = A[L.A_MM] / ns - Mb * Mb
+    U4 = (A[L.A_M2] / ns) * (A[L.A_M2] / ns) / (A[L.A_M4] / ns)
+    vals = three(L.A_SI) + three(L.A_SJ) + [A[L.A_SIJ] / ns, auto, A[L.A_E] / ns, A[L.A_E2] / ns, U4] + \
+        three(L.A_SIR) + three(L.A_SJR) + [A[L.A_SIJR] / ns, A[L.A_ER] / ns, A[L.A_ER2] / ns] + \
+        [A[L.A_ZI] / ns, A[L.A_ZJ] / ns, A[L.A_ZT] / ns, A[L.A_HI] / ns, A[L.A_HJ] / ns, A[L.A_HT] / ns,
+         A[L.A_Q] / ns if nc == 3 else 0.0]
+    if not has_rnorm:
+        # no block-spin sites: the reference divides 0/0 (heisenbergLib.c:784-790)
+        for k in range(11, 20):
+            vals[k] = float("nan")
+    return [float(v) for v in vals]
+
+
+def on_initial_spins(S: np.ndarray, nc: int, flunc: float = 0.0, seed: int = DEFAULT_SEED) -> np.ndarray:
+    """Initial O(n) spins: along x with length S (sign kept), tilted by a
+    random vector of length flunc and renormalised (heisenbergLib.c:160-172,
+    xyLib.c:120-132).  mcMain always passes flunc = 0."""
+    n = len(S)
+    init = np.zeros((n, nc))
+    init[:, 0] = S
+    if float(flunc) != 0.0:
+        rng = np.random.default_rng(seed)
+        r = rng.normal(size=(n, nc))
+        r /= np.linalg.norm(r, axis=1, keepdims=True)
+        v = init + float(flunc) * r
+        ln = np.linalg.norm(v, axis=1, keepdims=True)
+        v = np.where(ln >= 1e-5, v / np.where(ln > 0, ln, 1), v)
+        init = v * np.abs(S)[:, None]
+    return init
 
 
 def ising_main(algorithm, initSpin, nthermal, nsweep, ninterval, maxNLinking, nlink, linkStrength, linkedOrb,
@@ -90,16 +165,10 @@ def ising_main(algorithm, initSpin, nthermal, nsweep, ninterval, maxNLinking, nl
     maxl = int(maxNLinking)
     nl = np.asarray(nlink, np.int64)
     nbr = np.asarray(linkedOrb, np.int64).reshape(n, maxl)
-    flat = np.asarray(linkStrength, np.float64)
-    if flat.size != n * maxl or nl.size != n:
+    if len(linkStrength) != n * maxl or nl.size != n:
         raise ValueError("inconsistent tuple lengths")
-    lmask = np.arange(maxl)[None, :] < nl[:, None]
-    nbr = np.where(lmask, nbr, -1)
-    # coupling of link l of site i is entry i + l of the flat table (isingLib.c:30)
-    idx = np.arange(n)[:, None] + np.arange(maxl)[None, :]
-    if per_link_couplings:
-        idx = np.arange(n)[:, None] * maxl + np.arange(maxl)[None, :]
-    Jeff = np.where(lmask, flat[np.clip(idx, 0, flat.size - 1)], 0.0)
+    nbr = np.where(np.arange(maxl)[None, :] < nl[:, None], nbr, -1)
+    Jeff = ising_couplings(linkStrength, n, maxl, nl, per_link_couplings)
     pairs = np.asarray(corrOrbPair, np.int64).reshape(-1, 2)
     nr = len(rOrb)
     rclu = np.asarray(rOrbCluster, np.int64).reshape(nr, -1) if nr else None
@@ -109,29 +178,29 @@ def ising_main(algorithm, initSpin, nthermal, nsweep, ninterval, maxNLinking, nl
     wolff = int(algorithm) == 1
     alg = L.WOLFF if wolff else L.METROPOLIS
     nth, spf, ncl = _plan_counts(n, int(nthermal), int(nsweep), int(ninterval), wolff)
-    meas = L.MEAS_PAIRS | (L.MEAS_RNORM if nr else 0) | (0 if wolff else L.ISING_RELE)
+    meas = ising_meas_flags(nr > 0, wolff)
     # sigma = +1 everywhere: the spin of site i is sigma_i * initSpin[i]
     b = Batch(1, [g], [ReplicaSpec(0, 0)], [1.0], [float(h)], [np.ones(n)], meas=meas, nbins=1,
               nframes=int(nsweep), seed=seed, wolff=wolff)
     b.init()
-    frames = _run_frames(b, nth, int(nsweep), spf, alg, ncl, int(spinFrame), 1)
-    A = b.accumulators()[0]
-    ns = float(nsweep)
-    nLat = len(pairs)
-    # spin_tot: reference totSpin starts at sum of the first ninterval spins
-    s0 = S  # initial spins (+S)
-    off = s0[: min(int(ninterval), n)].sum() - s0.sum()
-    tot = A[L.A_TOT] / ns + off
-    if wolff:
-        tot += -A[L.A_LAST] / ns + s0[n - 1]
-    U4 = (A[L.A_M2] / ns) * (A[L.A_M2] / ns) / (A[L.A_M4] / ns)
-    out = (A[L.A_SI] / ns, A[L.A_SJ] / ns, A[L.A_SIJ] / ns, A[L.A_MM] / ns - (A[L.A_M] / ns) * (A[L.A_M] / ns),
-           A[L.A_E] / ns, A[L.A_E2] / ns, A[L.A_ER] / ns, A[L.A_ER2] / ns, U4, tot / nLat)
+    frames = _run_frames(b, nth, int(nsweep), spf, alg, ncl, int(spinFrame))[0]
+    out = ising_observables(b.accumulators()[0], int(nsweep), S, int(ninterval), wolff, len(pairs))
     if int(spinFrame) > 0:
         fr = tuple(tuple(float(v) for v in (f[:, 0] * S)) for f in frames)
     else:
         fr = 0.0
     return tuple(float(v) for v in out) + (fr,)
+
+
+def ising_meas_flags(has_rnorm: bool, wolff: bool) -> int:
+    """Measurement flags of the Ising result: pairs, block spins, and the
+    Metropolis energy relative to the end of thermalisation (isingLib.c:359)."""
+    return L.MEAS_PAIRS | (L.MEAS_RNORM if has_rnorm else 0) | (0 if wolff else L.ISING_RELE)
+
+
+def on_meas_flags(nc: int, has_circuits: bool, has_rnorm: bool, has_groups: bool) -> int:
+    return L.MEAS_PAIRS | (L.MEAS_Q if (nc == 3 and has_circuits) else 0) | \
+        (L.MEAS_RNORM if has_rnorm else 0) | (L.MEAS_GROUPS if has_groups else 0)
 
 
 def on_main(nc: int, algorithm, initSpin, initD, nthermal, nsweep, ninterval, maxNLinking, nlink, linkStrength,
@@ -165,47 +234,22 @@ def on_main(nc: int, algorithm, initSpin, initD, nthermal, nsweep, ninterval, ma
             groups.append(m)
     g = compile_graph(nc, nbr, J, S, D, pairs=pairs, circuits=circ, rOrb=rOrb, rOrbCluster=rclu, rlink=rl,
                       groups=groups, diag_only=int(ignoreNonDiagonalJ) > 0)
-    # initial state: along x with |S| (sign of S kept), tilted by flunc
-    # (heisenbergLib.c:160-172)
-    init = np.zeros((n, nc))
-    init[:, 0] = S
-    if float(flunc) != 0.0:
-        rng = np.random.default_rng(seed)
-        r = rng.normal(size=(n, nc))
-        r /= np.linalg.norm(r, axis=1, keepdims=True)
-        v = init + float(flunc) * r
-        ln = np.linalg.norm(v, axis=1, keepdims=True)
-        v = np.where(ln >= 1e-5, v / np.where(ln > 0, ln, 1), v)
-        init = v * np.abs(S)[:, None]
+    init = on_initial_spins(S, nc, float(flunc), seed)
     wolff = int(algorithm) == 1
     alg = L.WOLFF if wolff else L.METROPOLIS
     nth, spf, ncl = _plan_counts(n, int(nthermal), int(nsweep), int(ninterval), wolff)
-    meas = L.MEAS_PAIRS | (L.MEAS_Q if (nc == 3 and circ is not None and len(circ)) else 0) | \
-        (L.MEAS_RNORM if nr else 0) | (L.MEAS_GROUPS if ng > 0 else 0)
+    meas = on_meas_flags(nc, circ is not None and len(circ) > 0, nr > 0, ng > 0)
     b = Batch(nc, [g], [ReplicaSpec(0, 0)], [1.0], [float(h)], [init], store=store, meas=meas, nbins=1,
               nframes=int(nsweep), seed=seed, wolff=wolff)
     b.init()
-    frames = _run_frames(b, nth, int(nsweep), spf, alg, ncl, int(spinFrame), nc)
-    A = b.accumulators()[0]
-    ns = float(nsweep)
-    three = (lambda base: [A[base] / ns, A[base + 1] / ns, A[base + 2] / ns if nc == 3 else 0.0])
-    Mb = A[L.A_M] / ns
-    auto = A[L.A_MM] / ns - Mb * Mb
-    U4 = (A[L.A_M2] / ns) * (A[L.A_M2] / ns) / (A[L.A_M4] / ns)
-    vals = three(L.A_SI) + three(L.A_SJ) + [A[L.A_SIJ] / ns, auto, A[L.A_E] / ns, A[L.A_E2] / ns, U4] + \
-        three(L.A_SIR) + three(L.A_SJR) + [A[L.A_SIJR] / ns, A[L.A_ER] / ns, A[L.A_ER2] / ns] + \
-        [A[L.A_ZI] / ns, A[L.A_ZJ] / ns, A[L.A_ZT] / ns, A[L.A_HI] / ns, A[L.A_HJ] / ns, A[L.A_HT] / ns,
-         A[L.A_Q] / ns if nc == 3 else 0.0]
-    if not nr:
-        # no block-spin sites: the reference divides 0/0 (heisenbergLib.c:784-790)
-        for k in range(11, 20):
-            vals[k] = float("nan")
+    frames = _run_frames(b, nth, int(nsweep), spf, alg, ncl, int(spinFrame))[0]
+    vals = on_observables(b.accumulators()[0], int(nsweep), nc, nr > 0)
     if int(spinFrame) > 0:
         fr = tuple(tuple((float(v[0]), float(v[1]), float(v[2]) if nc == 3 else 0.0) for v in f) for f in frames)
     else:
         fr = 0.0
     if ng > 0:
-        gr = tuple(float(v) / ns for v in b.group_acc()[0])
+        gr = tuple(float(v) / float(nsweep) for v in b.group_acc()[0])
     else:
         gr = 0.0
-    return tuple(float(v) for v in vals) + (fr, gr)
+    return tuple(vals) + (fr, gr)

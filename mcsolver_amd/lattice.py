@@ -261,6 +261,75 @@ def build_lattice(Lx: int, Ly: int, Lz: int, LMatrix, pos, bonds: Sequence[Bond]
                    clu.astype(np.int32), rlink, circuits, groups, posv, cellv, orbv)
 
 
+def dipolar_bonds(lat: Lattice, alpha: float, cutoff: Optional[float] = None, ising: bool = False,
+                  max_links: int = 1 << 24) -> Lattice:
+    """Dipolar couplings as extra bond entries, truncated at |r| <= cutoff.
+
+    What Lattice.generateDipoleBondings (Lattice.py:286-308) sets out to add:
+    for every ordered pair of distinct sites (i, j), with r = r_i - r_j the
+    Cartesian separation and open boundaries (no periodic images), a link
+    i -> j of strength alpha/|r|^3 (Ising: only this isotropic part) or of
+    tensor alpha/|r|^3 (I - 3 r^ r^T) (O(n)), appended after the site's other
+    links even when i already links j.  The reference's call lacks an
+    argument and raises TypeError, and its O(n) branch exits, so only the
+    intent can be followed.  cutoff=None keeps every pair (N(N-1) links);
+    the energy is then the full open-boundary dipole sum
+    sum_{i<j} alpha/|r|^3 (s_i.s_j - 3 (s_i.r^)(s_j.r^)).
+    """
+    n = lat.n
+    pos = np.asarray(lat.pos, np.float64)
+    rc2 = np.inf if cutoff is None else float(cutoff) ** 2
+    src_l, tgt_l, j_l = [], [], []
+    total = 0
+    chunk = max(1, (1 << 22) // max(n, 1))
+    for a in range(0, n, chunk):
+        r = pos[a:a + chunk, None, :] - pos[None, :, :]          # [c, n, 3]
+        d2 = np.einsum("ijk,ijk->ij", r, r)
+        ii = np.arange(a, min(a + chunk, n))
+        ok = d2 <= rc2
+        ok[np.arange(len(ii)), ii] = False                      # no self link
+        si, tj = np.nonzero(ok)
+        if si.size == 0:
+            continue
+        total += si.size
+        if total > max_links:
+            raise ValueError("dipolar couplings would add %d+ links; give a cutoff" % total)
+        rv = r[si, tj]
+        dd = np.sqrt(d2[si, tj])
+        a3 = alpha / dd ** 3
+        j9 = np.zeros((si.size, 9))
+        if ising:
+            j9[:, 0] = a3
+        else:
+            u = rv / dd[:, None]
+            m = a3[:, None, None] * (np.eye(3)[None] - 3.0 * u[:, :, None] * u[:, None, :])
+            j9[:, 0], j9[:, 1], j9[:, 2] = m[:, 0, 0], m[:, 1, 1], m[:, 2, 2]
+            j9[:, 3], j9[:, 4], j9[:, 5] = m[:, 0, 1], m[:, 0, 2], m[:, 1, 2]
+            j9[:, 6], j9[:, 7], j9[:, 8] = m[:, 1, 0], m[:, 2, 0], m[:, 2, 1]
+        src_l.append(ii[si]); tgt_l.append(tj); j_l.append(j9)
+    if not src_l:
+        return lat
+    site = np.concatenate(src_l); other = np.concatenate(tgt_l); jj = np.concatenate(j_l)
+    order = np.lexsort((other, site))                            # target order within a site
+    site, other, jj = site[order], other[order], jj[order]
+    extra = np.bincount(site, minlength=n)
+    nlink = lat.nlink.astype(np.int64) + extra
+    maxl = int(nlink.max())
+    nbr = -np.ones((n, maxl), np.int32)
+    J = np.zeros((n, maxl, 9))
+    nbr[:, :lat.maxl] = lat.nbr
+    J[:, :lat.maxl] = lat.J
+    start = np.concatenate([[0], np.cumsum(extra)[:-1]])
+    slot = lat.nlink[site] + (np.arange(len(site)) - start[site])
+    nbr[site, slot] = other
+    J[site, slot] = jj
+    rlink = -np.ones((lat.rlink.shape[0], maxl), np.int32)
+    rlink[:, :lat.rlink.shape[1]] = lat.rlink
+    out = Lattice(**{k: getattr(lat, k) for k in lat.__dataclass_fields__})
+    out.nlink, out.nbr, out.J, out.rlink = nlink.astype(np.int32), nbr, J, rlink
+    return out
+
+
 def clamp_T(T: float) -> float:
     """mcMain.py:21 — temperatures below 0.1 run at 0.1."""
     return 0.1 if T < 0.1 else T
