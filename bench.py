@@ -5,16 +5,26 @@ every sweep, FP32 spins, FP64 accumulators).
 
 One step = one sweep of every replica (2,166,784 spin updates) + one
 tempering swap pass + one measured frame.  Printed as one JSON line with:
-  value        whole-job spin updates/s (inputs resident in HBM)
-  roofline     dominant kernel (k_lds) vs the 8 TB/s HBM peak, from
-               HIP-event timing of every launch of it in the timed region and
-               2 x (state bytes) algorithmic traffic per site update
+  value        whole-job spin updates/s (inputs resident in HBM), timed with
+               two events around the K steps
+  roofline     dominant kernel (k_lds) vs the 8 TB/s HBM peak: a second pass
+               of K/4 steps stamps every launch of it with HIP events (from
+               its own dispatch), and 2 x (state bytes) per site update is
+               the algorithmic traffic; `traffic` is the HBM bytes per
+               launch that rocprofv3 PMC counters measured for this kernel
+               and workload (committed summary under profiles/)
   cpu_baseline the FP64 reference-algorithm restatement (oracle), random
                sequential Metropolis with glibc rand(), on the host cores
   plain        the same trajectory with one launch per replica per colour per
                sweep (global-memory stencil) and the resulting speedup
 
 python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-plain]
+               [--workload c0|c1|c2|c3|c4] [--store f32|f16]
+
+--workload selects one of the other reference workloads (C0 Ising 32^2 x 16
+T; C1 Heisenberg honeycomb 64x64x2 x 64 T; C3 triangular DMI 128^2 x 1024
+(T, B); C4 cubic 64^3 x 128 T) and prints the same fields for it (without
+the plain and CPU legs); the default is the headline C2 scan.
 """
 from __future__ import annotations
 
@@ -31,6 +41,21 @@ sys.path.insert(0, ROOT)
 
 METRIC = "spin updates/s (replica·sites/s) + % HBM roofline; speedup vs plain at 2σ U4"
 HBM_PEAK = 8.0e12
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lds_xy_c2_hbm_traffic.md")
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the C2 sweep kernel measured with rocprofv3
+    PMC counters (FETCH_SIZE, WRITE_SIZE, each scaled by a copy-kernel
+    calibration), read from the committed profile summary."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def _cpu_worker(args):
@@ -78,6 +103,10 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
+    ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c3", "c4"])
+    ap.add_argument("--store", default="f32", choices=["f32", "f16"])
+    ap.add_argument("--parts", type=int, default=0,
+                    help="workgroups per LDS replica: 0 = balance automatically, 1 = never split")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -85,6 +114,9 @@ def main():
               "so N>1 is the same kernel on 1/N of the batch; multi-GPU runs are not part of this build.",
               file=sys.stderr)
         sys.exit(2)
+    if a.workload != "c2":
+        other_workload(a)
+        return
     # CPU baseline first (forks before the GPU is initialised)
     cpu = None if a.no_cpu else cpu_baseline()
 
@@ -104,13 +136,14 @@ def main():
     wl = W.build("c2", nframes=K + Wm + KT, nbins=16)
     b = wl.batch
     b.init()
-    b.run(0, Wm, 1, pt_every=wl.pt_every, block=a.block)
+    b.run(0, Wm, 1, pt_every=wl.pt_every, block=a.block, parts=a.parts)
     torch.cuda.synchronize()
 
     def plan(nsw, sweep0, timing=None):
         p = L.Plan()
         p.algorithm, p.nthermal, p.nsweep, p.sweeps_per_frame, p.pt_every = L.METROPOLIS, 0, nsw, 1, wl.pt_every
         p.kernel, p.block, p.sweep0, p.frame0 = L.KERNEL_AUTO, a.block, sweep0, sweep0
+        p.parts = a.parts
         if timing is not None:
             p.timing = C.cast(timing, C.POINTER(C.c_double))
         return p
@@ -137,7 +170,7 @@ def main():
     k_ms, k_launch, k_sweeps, k_upd = timing[0], timing[1], timing[2], timing[3]
     achieved = (k_upd * bpu) / (k_ms / 1e3) if k_ms > 0 else 0.0
     roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "k_lds<2,float>",
+            "frac": achieved / HBM_PEAK, "traffic": profiled_traffic(), "kernel": "k_lds<2,float>",
             "avg_launch_us": 1e3 * k_ms / max(k_launch, 1), "launches": int(k_launch),
             "algorithmic_bytes_per_launch": bpu * k_upd / max(k_launch, 1)}
     acc = b.accumulators()
@@ -162,6 +195,70 @@ def main():
         out["plain"] = {"value": pv, "unit": "spin updates/s", "steps": a.plain_steps,
                         "kernel": "k_pass, one launch per replica per colour per sweep + per-replica measure"}
         out["speedup_vs_plain"] = value / pv
+    print(json.dumps(out), flush=True)
+
+
+def other_workload(a):
+    """One of C0, C1, C3, C4: K measured sweeps of every replica (no
+    tempering, so one launch runs all sweeps on the LDS path), then a K/4
+    sweep pass with per-launch HIP events for the roofline."""
+    import torch
+    from mcsolver_amd import _lib as L
+    from mcsolver_amd import workloads as W
+
+    torch.cuda.set_device(0)
+    K, Wm = a.steps, a.warmup
+    KT = max(1, K // 4)
+    meas = L.MEAS_PAIRS | (L.MEAS_Q if a.workload == "c3" else 0)
+    wl = W.build(a.workload, store=a.store, nframes=K + Wm + KT, nbins=16, meas=meas)
+    b = wl.batch
+    b.init()
+    b.run(0, Wm, 1, parts=a.parts)
+    torch.cuda.synchronize()
+
+    def plan(nsw, sweep0, timing=None):
+        p = L.Plan()
+        p.algorithm, p.nthermal, p.nsweep, p.sweeps_per_frame = L.METROPOLIS, 0, nsw, 1
+        p.kernel, p.block, p.sweep0, p.frame0 = L.KERNEL_AUTO, a.block, sweep0, sweep0
+        p.parts = a.parts
+        if timing is not None:
+            p.timing = C.cast(timing, C.POINTER(C.c_double))
+        return p
+
+    st, en = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    p = plan(K, Wm)
+    torch.cuda.synchronize()
+    st.record()
+    L.check(L.lib().mcg_run(C.byref(b.b), C.byref(p), b._stream()), "mcg_run")
+    en.record()
+    torch.cuda.synchronize()
+    ms = st.elapsed_time(en)
+    ups = wl.updates_per_sweep
+    value = ups * K / (ms / 1e3)
+    timing = (C.c_double * 4)()
+    pt = plan(KT, Wm + K, timing)
+    L.check(L.lib().mcg_run(C.byref(b.b), C.byref(pt), b._stream()), "mcg_run")
+    torch.cuda.synchronize()
+    bpu = W.bytes_per_update(wl.nc, wl.store)
+    k_ms, k_launch, k_sweeps, k_upd = timing[0], timing[1], timing[2], timing[3]
+    stream = k_launch > k_sweeps  # one launch per colour: the streaming pass
+    ty = {"f32": "float", "f16": "__half", "i8": "signed char"}[wl.store]
+    kern = ("k_pass<%d,%s>" if stream else "k_lds<%d,%s>") % (wl.nc, ty)
+    achieved = (k_upd * bpu) / (k_ms / 1e3) if k_ms > 0 else 0.0
+    acc = b.accumulators()
+    assert b.error_flag() == 0
+    assert (acc[:, L.A_COUNT] == K + Wm + KT).all()
+    out = {"metric": METRIC, "value": value, "unit": "spin updates/s", "n_gpus": 1, "steps": K, "warmup": Wm,
+           "ms_per_step": ms / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": {"f32": "f32", "f16": "f32 (f16 storage)", "i8": "f32 (int8 spins)"}[wl.store],
+           "data": "synthetic",
+           "config": {"workload": wl.name + ": " + wl.description, "replicas": len(b.reps), "spins_per_sweep": ups,
+                      "measure_every": 1, "spin_storage": wl.store, "accumulators": "fp64"},
+           "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK, "traffic": None, "kernel": kern,
+                        "avg_launch_us": 1e3 * k_ms / max(k_launch, 1), "launches": int(k_launch),
+                        "algorithmic_bytes_per_launch": bpu * k_upd / max(k_launch, 1)},
+           "cpu_baseline": None}
     print(json.dumps(out), flush=True)
 
 

@@ -152,6 +152,11 @@ typedef struct mcg_plan {
   int32_t reps_per_block;  /* replicas packed per LDS block (0 = auto) */
   int32_t clusters_per_sweep; /* Wolff: clusters per replica per sweep (0 = n) */
   int32_t frame0;          /* index of the first measured frame of this call */
+  int32_t parts;           /* LDS kernel: workgroups per replica (0 = auto: split replicas of
+                              8192+ sites that exceed one CU's share of the batch; 1 = never
+                              split; 2..4 = split every LDS replica).  A launch with split
+                              replicas runs one sweep. */
+  int32_t pad_;
   int64_t sweep0;          /* global sweep index of the first sweep (Philox counter) */
   double *timing;          /* host double[4] or NULL: when set, mcg_run brackets every
                               launch of the sweep kernel with HIP events on the stream

@@ -49,13 +49,14 @@ class Batch(C.Structure):
 
 class Plan(C.Structure):
     _fields_ = [(k, C.c_int32) for k in ("algorithm", "nthermal", "nsweep", "sweeps_per_frame", "pt_every", "kernel",
-                                         "block", "reps_per_block", "clusters_per_sweep", "frame0")] + \
+                                         "block", "reps_per_block", "clusters_per_sweep", "frame0", "parts",
+                                         "pad_")] + \
                [("sweep0", C.c_int64), ("timing", C.POINTER(C.c_double))]
 
 
 assert C.sizeof(Graph) == 48 + 17 * 8
 assert C.sizeof(Replica) == 40
-assert C.sizeof(Plan) == 56
+assert C.sizeof(Plan) == 64
 
 _lib = None
 

@@ -345,6 +345,7 @@ __device__ __forceinline__ float site_energy(const SV& sp, int i, const uint32_t
 // group reductions (a group = TPR consecutive threads, TPR a multiple of 64)
 // ---------------------------------------------------------------------------
 static constexpr int RK = 8;  // values per reduction call
+static constexpr int FRM_ACC = 96;  // frame scratch: accumulator staging starts here
 
 __device__ __forceinline__ void group_sum8(double* v, double* s_red, int g, int TPR) {
 #pragma unroll
@@ -437,14 +438,18 @@ __device__ __forceinline__ float signed_area(const float* a, const float* b, con
 // Measures one frame.  Tracked E and M (FP64) of the replica are passed in
 // st[]; everything else is computed from the spins.  Only the group leader
 // (lt == 0) writes the accumulators.
-template <int NC, typename SV>
+// Observables taken from the spins themselves (pairs, Q, block spins,
+// groups); a kernel compiled without them (FULL = false) is much smaller.
+static constexpr int MEAS_SPIN_OBS = MCG_MEAS_PAIRS | MCG_MEAS_Q | MCG_MEAS_RNORM | MCG_MEAS_GROUPS;
+
+template <int NC, typename SV, bool FULL = true>
 __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& B, const double* st, int row,
                               int64_t frame, uint32_t key0, uint32_t key1, float h) {
   const mcg_graph& G = *C.G;
   const int lt = C.lt, TPR = C.TPR;
   double* F = C.s_frm;  // leader-only use
   double v[RK];
-  const int flags = B.meas;
+  const int flags = FULL ? B.meas : (B.meas & ~MEAS_SPIN_OBS);
   // --- correlation pairs: S_i, S_j, sum s_i.s_j, last-site spin
   if ((flags & MCG_MEAS_PAIRS) && G.nlat > 0) {
     for (int k = 0; k < RK; ++k) v[k] = 0.0;
@@ -566,100 +571,120 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
       for (int k = 0; k < 6; ++k) F[32 + 3 * g0 + k] = v[k];
   }
 
-  if (lt != 0) return;
-  // --- leader: per-frame scalars -> FP64 accumulators (isingLib.c:388-431,
-  //     heisenbergLib.c:677-830 / xyLib.c:605-757)
-  double* A = B.acc + (size_t)row * MCG_NACC;
-  const double n = G.n;
-  const double nLat = G.nlat > 0 ? G.nlat : G.n;
-  const bool pairs = (flags & MCG_MEAS_PAIRS) && G.nlat > 0;
-  double E = st[MCG_S_E];
-  if (flags & MCG_ISING_RELE) E -= st[MCG_S_EREF];
-  const double e = E / n;
-  const double M0 = st[MCG_S_M], M1 = st[MCG_S_M + 1], M2 = st[MCG_S_M + 2];
-  double Mf;
-  const int ax = NC == 3 ? 2 : 0;
-  if constexpr (NC == 1) {
-    Mf = pairs ? F[0] / nLat : M0 / nLat;
-    if (pairs) {
-      A[MCG_A_SI] += fabs(F[0]) / nLat;
-      A[MCG_A_SJ] += fabs(F[3]) / nLat;
-      A[MCG_A_SIJ] += F[6] / nLat;
-      A[MCG_A_LAST] += F[7];
-    }
-    A[MCG_A_TOT] += M0;
-  } else {
-    const double Mv[3] = {M0, M1, NC == 3 ? M2 : 0.0};
-    double dir[3] = {Mv[0], Mv[1], Mv[2]};
-    const double ml = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
-    if (ml >= 1e-5) {
-      dir[0] /= ml; dir[1] /= ml; dir[2] /= ml;
-    }
-    if (NC == 3 || !pairs)
-      Mf = ml / nLat;
-    else
-      Mf = sqrt(F[0] * F[0] + F[1] * F[1]) / nLat;
-    if (pairs) {
-      for (int k = 0; k < NC; ++k) {
-        A[MCG_A_SI + k] += fabs(F[k] / nLat);
-        A[MCG_A_SJ + k] += fabs(F[3 + k] / nLat);
+  // --- leader: this frame's contribution to every FP64 accumulator of the
+  //     row (isingLib.c:388-431, heisenbergLib.c:677-830 / xyLib.c:605-757),
+  //     staged in LDS; the group's threads then apply them to global memory
+  //     in parallel (one read-modify-write each instead of a serial chain)
+  double* Cn = F + FRM_ACC;  // [0, NACC) accumulators, [NACC, +NBINQ) bin, then bin index
+  const int ngq = (NC > 1 && (flags & MCG_MEAS_GROUPS)) ? G.ngroup : 0;
+  if (lt == 0) {
+    for (int q = 0; q < MCG_NACC + MCG_NBINQ + 1; ++q) Cn[q] = 0.0;
+    const double n = G.n;
+    const double nLat = G.nlat > 0 ? G.nlat : G.n;
+    const bool pairs = (flags & MCG_MEAS_PAIRS) && G.nlat > 0;
+    double E = st[MCG_S_E];
+    if (flags & MCG_ISING_RELE) E -= st[MCG_S_EREF];
+    const double e = E / n;
+    const double M0 = st[MCG_S_M], M1 = st[MCG_S_M + 1], M2 = st[MCG_S_M + 2];
+    double Mf;
+    const int ax = NC == 3 ? 2 : 0;
+    if constexpr (NC == 1) {
+      Mf = pairs ? F[0] / nLat : M0 / nLat;
+      if (pairs) {
+        Cn[MCG_A_SI] = fabs(F[0]) / nLat;
+        Cn[MCG_A_SJ] = fabs(F[3]) / nLat;
+        Cn[MCG_A_SIJ] = F[6] / nLat;
+        Cn[MCG_A_LAST] = F[7];
       }
-      A[MCG_A_SIJ] += F[6] / nLat;
-      A[MCG_A_ZI] += (dir[0] * F[0] + dir[1] * F[1] + dir[2] * F[2]) / nLat;
-      A[MCG_A_ZJ] += (dir[0] * F[3] + dir[1] * F[4] + dir[2] * F[5]) / nLat;
-      A[MCG_A_HI] += F[ax] / nLat;
-      A[MCG_A_HJ] += F[3 + ax] / nLat;
-    }
-    A[MCG_A_ZT] += (dir[0] * Mv[0] + dir[1] * Mv[1] + dir[2] * Mv[2]) / nLat;
-    A[MCG_A_HT] += Mv[ax] / nLat;
-    if ((flags & MCG_MEAS_Q) && NC == 3 && G.ncirc > 0) A[MCG_A_Q] += F[8] / REF_PI / 4;
-    if ((flags & MCG_MEAS_RNORM) && G.nr > 0) {
-      for (int k = 0; k < NC; ++k) {
-        A[MCG_A_SIR + k] += fabs(F[16 + k] / F[24]);
-        A[MCG_A_SJR + k] += fabs(F[19 + k] / F[25]);
+      Cn[MCG_A_TOT] = M0;
+    } else {
+      const double Mv[3] = {M0, M1, NC == 3 ? M2 : 0.0};
+      double dir[3] = {Mv[0], Mv[1], Mv[2]};
+      const double ml = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+      if (ml >= 1e-5) {
+        dir[0] /= ml; dir[1] /= ml; dir[2] /= ml;
       }
-      A[MCG_A_SIJR] += F[22] / F[26];
-    }
-    // groups
-    if (ng > 0) {
-      double* GA = B.gacc + (size_t)row * ((ng + 2) * (ng + 1));
-      double Gv[17][3];
-      for (int a = 0; a < ng; ++a)
-        for (int k = 0; k < 3; ++k) Gv[a][k] = F[32 + 3 * a + k];
-      for (int k = 0; k < 3; ++k) Gv[ng][k] = Mv[k] / nLat;
-      for (int a = 0; a <= ng; ++a)
-        for (int b2 = 0; b2 <= ng; ++b2) {
-          const double d = Gv[a][0] * Gv[b2][0] + Gv[a][1] * Gv[b2][1] + Gv[a][2] * Gv[b2][2];
-          GA[a * (ng + 1) + b2] += d;
-          if (a == b2) GA[(ng + 1) * (ng + 1) + a] += d * d;
+      if (NC == 3 || !pairs)
+        Mf = ml / nLat;
+      else
+        Mf = sqrt(F[0] * F[0] + F[1] * F[1]) / nLat;
+      if (pairs) {
+        for (int k = 0; k < NC; ++k) {
+          Cn[MCG_A_SI + k] = fabs(F[k] / nLat);
+          Cn[MCG_A_SJ + k] = fabs(F[3 + k] / nLat);
         }
+        Cn[MCG_A_SIJ] = F[6] / nLat;
+        Cn[MCG_A_ZI] = (dir[0] * F[0] + dir[1] * F[1] + dir[2] * F[2]) / nLat;
+        Cn[MCG_A_ZJ] = (dir[0] * F[3] + dir[1] * F[4] + dir[2] * F[5]) / nLat;
+        Cn[MCG_A_HI] = F[ax] / nLat;
+        Cn[MCG_A_HJ] = F[3 + ax] / nLat;
+      }
+      Cn[MCG_A_ZT] = (dir[0] * Mv[0] + dir[1] * Mv[1] + dir[2] * Mv[2]) / nLat;
+      Cn[MCG_A_HT] = Mv[ax] / nLat;
+      if ((flags & MCG_MEAS_Q) && NC == 3 && G.ncirc > 0) Cn[MCG_A_Q] = F[8] / REF_PI / 4;
+      if ((flags & MCG_MEAS_RNORM) && G.nr > 0) {
+        for (int k = 0; k < NC; ++k) {
+          Cn[MCG_A_SIR + k] = fabs(F[16 + k] / F[24]);
+          Cn[MCG_A_SJR + k] = fabs(F[19 + k] / F[25]);
+        }
+        Cn[MCG_A_SIJR] = F[22] / F[26];
+      }
+      // groups: the total (all sites / nLat) is group ng
+      if (ngq > 0)
+        for (int k = 0; k < 3; ++k) F[32 + 3 * ngq + k] = Mv[k] / nLat;
+    }
+    if ((flags & MCG_MEAS_RNORM) && G.nr > 0) {
+      const double er = (F[27] / 2 + F[28]) / G.nr;
+      Cn[MCG_A_ER] = er;
+      Cn[MCG_A_ER2] = er * er;
+    }
+    Cn[MCG_A_E] = e;
+    Cn[MCG_A_E2] = e * e;
+    Cn[MCG_A_M] = Mf;
+    Cn[MCG_A_M2] = Mf * Mf;
+    Cn[MCG_A_M4] = Mf * Mf * Mf * Mf;
+    Cn[MCG_A_MM] = Mf;  // applied as A[MM] += A[MPREV] * Mf; A[MPREV] = Mf
+    Cn[MCG_A_COUNT] = 1.0;
+    Cn[MCG_A_ABSM] = fabs(Mf);
+    // bins for error bars
+    int64_t nf = B.nframes > 0 ? B.nframes : 1;
+    int64_t bin = (frame * (int64_t)B.nbins) / nf;
+    if (bin >= B.nbins) bin = B.nbins - 1;
+    if (bin < 0) bin = 0;
+    Cn[MCG_NACC + 0] = e;
+    Cn[MCG_NACC + 1] = e * e;
+    Cn[MCG_NACC + 2] = Mf;
+    Cn[MCG_NACC + 3] = Mf * Mf;
+    Cn[MCG_NACC + 4] = Mf * Mf * Mf * Mf;
+    Cn[MCG_NACC + MCG_NBINQ] = (double)bin;
+  }
+  __syncthreads();
+  double* A = B.acc + (size_t)row * MCG_NACC;
+  const int nge = ngq > 0 ? (ngq + 1) * (ngq + 1) : 0;
+  for (int q = lt; q < MCG_NACC + MCG_NBINQ + nge; q += TPR) {
+    if (q < MCG_NACC) {
+      if (q == MCG_A_MPREV) continue;
+      if (q == MCG_A_MM) {
+        const double prev = A[MCG_A_MPREV];
+        A[MCG_A_MM] += prev * Cn[MCG_A_MM];
+        A[MCG_A_MPREV] = Cn[MCG_A_MM];
+      } else {
+        A[q] += Cn[q];
+      }
+    } else if (q < MCG_NACC + MCG_NBINQ) {
+      const int64_t bin = (int64_t)Cn[MCG_NACC + MCG_NBINQ];
+      B.bins[((size_t)row * B.nbins + bin) * MCG_NBINQ + (q - MCG_NACC)] += Cn[q];
+    } else {
+      // orbital groups (heisenbergLib.c:805-830): <g_a . g_b> and <(g_a . g_a)^2>
+      const int j = q - MCG_NACC - MCG_NBINQ, ga = j / (ngq + 1), gb = j % (ngq + 1);
+      double* GA = B.gacc + (size_t)row * ((ngq + 2) * (ngq + 1));
+      const double* va = F + 32 + 3 * ga;
+      const double* vb = F + 32 + 3 * gb;
+      const double d = va[0] * vb[0] + va[1] * vb[1] + va[2] * vb[2];
+      GA[j] += d;
+      if (ga == gb) GA[(ngq + 1) * (ngq + 1) + ga] += d * d;
     }
   }
-  if ((flags & MCG_MEAS_RNORM) && G.nr > 0) {
-    const double er = (F[27] / 2 + F[28]) / G.nr;
-    A[MCG_A_ER] += er;
-    A[MCG_A_ER2] += er * er;
-  }
-  A[MCG_A_E] += e;
-  A[MCG_A_E2] += e * e;
-  A[MCG_A_M] += Mf;
-  A[MCG_A_M2] += Mf * Mf;
-  A[MCG_A_M4] += Mf * Mf * Mf * Mf;
-  A[MCG_A_MM] += A[MCG_A_MPREV] * Mf;
-  A[MCG_A_MPREV] = Mf;
-  A[MCG_A_COUNT] += 1.0;
-  A[MCG_A_ABSM] += fabs(Mf);
-  // bins for error bars
-  int64_t nf = B.nframes > 0 ? B.nframes : 1;
-  int64_t bin = (frame * (int64_t)B.nbins) / nf;
-  if (bin >= B.nbins) bin = B.nbins - 1;
-  if (bin < 0) bin = 0;
-  double* Bq = B.bins + ((size_t)row * B.nbins + bin) * MCG_NBINQ;
-  Bq[0] += e;
-  Bq[1] += e * e;
-  Bq[2] += Mf;
-  Bq[3] += Mf * Mf;
-  Bq[4] += Mf * Mf * Mf * Mf;
 }
 
 // full FP64 recomputation of E and M of one replica (group-parallel)
@@ -684,20 +709,39 @@ __device__ void recompute(const SV& sp, const mcg_graph& G, const float* coup, c
 // ---------------------------------------------------------------------------
 // k_lds: persistent LDS-resident sweeps
 // ---------------------------------------------------------------------------
+// One workgroup of the LDS kernel: nrep replicas of one graph, or one part
+// of a replica split over nparts workgroups (part, nparts, slot of its
+// counter and partial sums, offset of its extra-group table).
 struct WorkItem {
-  int32_t graph, nrep, off, pad;
+  int32_t graph, nrep, off;
+  uint8_t part, nparts;
+  uint16_t slot;
+  int32_t hoff;
 };
+
+// what a workgroup needs of each of its replicas, resolved on the host so
+// that staging starts one dependent load after the work item
+struct ItemRep {
+  int64_t spin_off;
+  int32_t rep, serial, row0, pad;
+};
+
+static constexpr int MAX_PARTS = 4;
+static constexpr int SYNC_STRIDE = 16;  // uint32 per replica counter (one 64-byte line)
 
 struct SweepArgs {
   int64_t k_begin;      // run-relative index of the first sweep of this launch
   int64_t philox_base;  // added to k for the Philox counter
-  int32_t nsweeps;
+  int32_t nsweeps;      // 1 whenever the launch has split replicas
   int32_t nthermal;     // run-relative: sweeps < nthermal are thermal
   int32_t spf;          // sweeps per frame
   int32_t frame0;       // index of the first frame of the run
+  uint32_t* sync;       // split replicas: arrival counters, zero at run start
+  double* xpart;        // split replicas: per-part (dE, dM) partial sums
+  const int32_t* halo;  // split replicas: extra-group tables
 };
 
-static constexpr int FRM = 96;  // frame scratch doubles per group
+static constexpr int FRM = 96 + 48;  // frame scratch doubles per group: scalars, then accumulator staging
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
@@ -744,7 +788,95 @@ __host__ __device__ inline size_t lds_replica_bytes(int nc, int n, int ssize) {
   return align16((size_t)nc * n * ssize);
 }
 
-template <int NC, typename ST, typename SV>
+// Split replicas.  A replica too large for one CU's share of a sweep runs as
+// P workgroups (parts), each holding the whole replica in LDS.  Part p owns
+// the Philox groups [ngrp*p/P, ngrp*(p+1)/P) of every colour class.  Its own
+// sites need, when they are updated, the current values of their neighbours;
+// those of lower colour that another part owns are recomputed here too: the
+// host lists, per part and colour, the extra groups that close the own
+// sites under "neighbour of lower colour".  Every part starts the sweep from
+// the same spins and draws the same Philox words, so the recomputed sites
+// come out bit for bit as their owner computes them, and no data moves
+// between parts during a sweep.  A launch with split replicas does one
+// sweep.  At its end each part publishes its own sites and its partial
+// sums (dE, dM) at the device coherence point and counts itself on the
+// replica's counter; the part that arrives last reads the other parts' sites
+// back, combines the sums and does the measurement.  Nothing waits.
+//
+// Device coherence: the parts may sit on different XCDs, whose L2 caches are
+// not coherent with each other for ordinary accesses.  Published data goes
+// through device-scope (agent) atomic stores and loads, which the compiler
+// emits at the device coherence point (sc1), and each wave waits for its
+// stores to be acknowledged (s_waitcnt 0) before the workgroup barrier that
+// precedes the counter increment.
+__device__ __forceinline__ void part_range(int b, int cnt, int ngrp, int gd, int p, int np, int& e0, int& e1) {
+  const int qa = (int)((int64_t)ngrp * p / np), qb = (int)((int64_t)ngrp * (p + 1) / np);
+  e0 = b + qa * gd;
+  e1 = b + min(qb * gd, cnt);
+}
+
+template <typename T>
+__device__ __forceinline__ void dev_store(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T dev_load(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// every store this wave issued has been acknowledged
+__device__ __forceinline__ void stores_done() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+template <int S>
+struct RawBits;
+template <>
+struct RawBits<1> {
+  typedef uint8_t T;
+};
+template <>
+struct RawBits<2> {
+  typedef uint16_t T;
+};
+template <>
+struct RawBits<4> {
+  typedef uint32_t T;
+};
+
+// copy the sites of part p (every colour) between LDS and global memory, as
+// raw bit patterns of the storage type (ordinary accesses; the caller
+// orders them with fences where another workgroup reads them)
+template <typename ST, bool OUT>
+__device__ void part_copy(ST* lds_s, ST* gsp, int n, int nc, const int32_t* colptr, int ncol, int gd, int p, int np,
+                          int lt, int TPR) {
+  typedef typename RawBits<sizeof(ST)>::T U;
+  U* lds = (U*)lds_s;
+  GAS U* g = (GAS U*)gas((U*)gsp);
+  for (int c = 0; c < ncol; ++c) {
+    const int b = colptr[c], cnt = colptr[c + 1] - b;
+    const int ngrp = (cnt + gd - 1) / gd;
+    int e0, e1;
+    part_range(b, cnt, ngrp, gd, p, np, e0, e1);
+    for (int k = 0; k < nc; ++k) {
+      U* l = lds + (size_t)k * n;
+      GAS U* q = g + (size_t)k * n;
+      for (int e = e0 + lt; e < e1; e += TPR) {
+        if constexpr (OUT) q[e] = l[e];
+        else l[e] = q[e];
+      }
+    }
+  }
+}
+
+struct PartInfo {
+  int part = 0, np = 1;            // this workgroup's part of the replica
+  const int32_t* hx = nullptr;     // extra groups: [2c] start, [2c+1] count per colour, into halo
+  const int32_t* halo = nullptr;
+};
+
+template <int NC, typename ST, bool FULL, typename SV>
 __device__ __forceinline__ void lds_sweep_tail(const SV& sv, const mcg_graph& G, const mcg_batch& B, const float* s_coup,
                                                const float* s_cls, double* s_red, double* st, FrameCtx& C, int g,
                                                int lt, int TPR, int row, float h, int64_t k, const SweepArgs& a,
@@ -774,7 +906,8 @@ __device__ __forceinline__ void lds_sweep_tail(const SV& sv, const mcg_graph& G,
   }
   if (therm_end && lt == 0) st[MCG_S_EREF] = st[MCG_S_E];
   __syncthreads();
-  if (meas) measure_frame<NC>(sv, C, B, st, row, (k - a.nthermal + 1) / a.spf - 1 + a.frame0, key0, key1, h);
+  if (meas)
+    measure_frame<NC, SV, FULL>(sv, C, B, st, row, (k - a.nthermal + 1) / a.spf - 1 + a.frame0, key0, key1, h);
   __syncthreads();
 }
 
@@ -783,16 +916,18 @@ __device__ __forceinline__ void lds_sweep_tail(const SV& sv, const mcg_graph& G,
 // global link table with couplings/classes read from HBM (> 256 tensors or
 // classes).  Each mode is its own instantiation so that every table pointer
 // has one address space.
-template <int NC, typename ST, int MODE>
+template <int NC, typename ST, int MODE, bool FULL>
 __device__ __forceinline__ void lds_sweeps(const mcg_graph& G, const mcg_batch& B, const SweepArgs& a,
                                            const Spins<ST>& sp, const float* coup,
                                            const float* clsdat, const int32_t* s_colptr, const uint8_t* s_pid,
                                            const int32_t* s_pat, double* s_red, double* st, FrameCtx& C, int g,
                                            int lt, int TPR, int row, float beta, float h, uint32_t key0,
-                                           uint32_t key1) {
+                                           uint32_t key1, const PartInfo& pi, double& dE, double* dM) {
   constexpr int GD = SitesPerDraw<NC>::G;
   const int zmax = G.zmax;
-  double dE = 0.0, dM[3] = {0.0, 0.0, 0.0};
+  const bool split = pi.np > 1;
+  dE = 0.0;
+  dM[0] = dM[1] = dM[2] = 0.0;
   for (int s = 0; s < a.nsweeps; ++s) {
     const int64_t k = a.k_begin + s;
     const uint64_t kk = (uint64_t)(a.philox_base + k);
@@ -800,18 +935,34 @@ __device__ __forceinline__ void lds_sweeps(const mcg_graph& G, const mcg_batch& 
     for (int c = 0; c < G.ncol; ++c) {
       const int b = s_colptr[c], cnt = s_colptr[c + 1] - b;
       const int ngrp = (cnt + GD - 1) / GD;
+      const int q0 = split ? (int)((int64_t)ngrp * pi.part / pi.np) : 0;
+      const int q1 = split ? (int)((int64_t)ngrp * (pi.part + 1) / pi.np) : ngrp;
+      // own groups, then (split) the other parts' groups this part needs,
+      // whose changes are not counted here
+      const int nx = split ? pi.hx[2 * c + 1] : 0;
+      const int32_t* xl = split ? pi.halo + pi.hx[2 * c] : nullptr;
+      float xE = 0.0f, xM[3] = {0.0f, 0.0f, 0.0f};
       if constexpr (MODE == 0) {
-        for (int q = lt; q < ngrp; q += TPR)
+        for (int q = q0 + lt; q < q1; q += TPR)
           metro_group_pat<NC, ST>(sp, b, cnt, q, c, s_pid, s_pat, zmax, coup, clsdat, beta, h, kk, key0, key1, fE,
                                   fM);
+        for (int j = lt; j < nx; j += TPR)
+          metro_group_pat<NC, ST>(sp, b, cnt, xl[j], c, s_pid, s_pat, zmax, coup, clsdat, beta, h, kk, key0, key1,
+                                  xE, xM);
       } else if constexpr (MODE == 1) {
-        for (int q = lt; q < ngrp; q += TPR)
+        for (int q = q0 + lt; q < q1; q += TPR)
           metro_group<NC>(sp, b, cnt, q, c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, coup, gas(G.cls),
                           clsdat, beta, h, kk, key0, key1, fE, fM);
+        for (int j = lt; j < nx; j += TPR)
+          metro_group<NC>(sp, b, cnt, xl[j], c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, coup, gas(G.cls),
+                          clsdat, beta, h, kk, key0, key1, xE, xM);
       } else {
-        for (int q = lt; q < ngrp; q += TPR)
+        for (int q = q0 + lt; q < q1; q += TPR)
           metro_group<NC>(sp, b, cnt, q, c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, gas(coup),
                           gas(G.cls), gas(clsdat), beta, h, kk, key0, key1, fE, fM);
+        for (int j = lt; j < nx; j += TPR)
+          metro_group<NC>(sp, b, cnt, xl[j], c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, gas(coup),
+                          gas(G.cls), gas(clsdat), beta, h, kk, key0, key1, xE, xM);
       }
       __syncthreads();
     }
@@ -819,29 +970,39 @@ __device__ __forceinline__ void lds_sweeps(const mcg_graph& G, const mcg_batch& 
     dM[0] += (double)fM[0];
     dM[1] += (double)fM[1];
     dM[2] += (double)fM[2];
+    if (split) continue;  // one sweep per launch; the last part to finish runs the tail
     const bool meas = k >= a.nthermal && ((k - a.nthermal + 1) % a.spf) == 0;
     const bool therm_end = (k + 1 == a.nthermal);
     const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
     if (meas || therm_end || rec || s == a.nsweeps - 1) {
-      lds_sweep_tail<NC, ST>(sp, G, B, coup, clsdat, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end, rec, dE,
+      lds_sweep_tail<NC, ST, FULL>(sp, G, B, coup, clsdat, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end,
+                                   rec, dE,
                              dM, key0, key1);
     }
   }
 }
 
-template <int NC, typename ST>
+// MODE: link mode of every graph of the launch (0 pattern, 1 global links
+// with LDS tables, 2 wide); FULL: spin-derived observables are measured.
+// Both are compile-time so that each instantiation carries only the code it
+// runs (instruction-cache misses dominate the per-launch cost otherwise).
+template <int NC, typename ST, int MODE, bool FULL>
 __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __restrict__ items,
-                                              const int32_t* __restrict__ item_reps, SweepArgs a) {
+                                              const ItemRep* __restrict__ recs, SweepArgs a) {
   const WorkItem w = items[blockIdx.x];
   const mcg_graph G = B.graphs[w.graph];
   const int R = w.nrep;
   const int TPR = blockDim.x / R;
   const int g = threadIdx.x / TPR;
   const int lt = threadIdx.x - g * TPR;
+  const ItemRep ir = recs[w.off + g];
+  const int rep = ir.rep;
+  const int row = ir.row0 + B.rung[rep];
+  const float beta = B.row_beta[row], h = B.row_h[row];
   const int nwaves = blockDim.x / 64;
-  const bool perm = G.npat > 0;
+  const bool perm = MODE == 0;
   const int n = G.n, zmax = G.zmax;
-  const bool wide = G.lcid != nullptr || G.ncls > 256;
+  const bool wide = MODE == 2;
   const int ncl = wide ? 0 : G.ncoup;  // couplings staged in LDS
   const int ncs = wide ? 0 : G.ncls;   // site classes staged in LDS
 
@@ -871,34 +1032,83 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
     for (int t = threadIdx.x; t < n; t += blockDim.x) s_pid[t] = G.pat[t];
   }
 
-  const int rep = item_reps[w.off + g];
-  const mcg_replica rp = B.reps[rep];
-  ST* gsp = (ST*)B.spins + rp.spin_off;
+  ST* gsp = (ST*)B.spins + ir.spin_off;
   ST* my = s_spin + lds_replica_bytes(NC, n, sizeof(ST)) / sizeof(ST) * g;
   stage<ST, true>(my, gsp, NC * n, lt, TPR);
   double* st = s_st + g * MCG_NSTATE;
   if (lt < MCG_NSTATE) st[lt] = B.state[(size_t)rep * MCG_NSTATE + lt];
   __syncthreads();
 
-  const uint32_t key0 = B.seed_lo, key1 = B.seed_hi ^ rp.serial;
-  const int row = rp.row0 + B.rung[rep];
-  const float beta = B.row_beta[row], h = B.row_h[row];
+  const uint32_t key0 = B.seed_lo, key1 = B.seed_hi ^ (uint32_t)ir.serial;
   const Spins<ST> sp{my, n};
-  if (perm) {
-    FrameCtx C{&G, s_coup, s_clsl, s_red, s_frm + g * FRM, g, lt, TPR};
-    lds_sweeps<NC, ST, 0>(G, B, a, sp, s_coup, s_clsl, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
-                          beta, h, key0, key1);
-  } else if (!wide) {
-    FrameCtx C{&G, s_coup, s_clsl, s_red, s_frm + g * FRM, g, lt, TPR};
-    lds_sweeps<NC, ST, 1>(G, B, a, sp, s_coup, s_clsl, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
-                          beta, h, key0, key1);
-  } else {
-    FrameCtx C{&G, G.coup, G.clsdat, s_red, s_frm + g * FRM, g, lt, TPR};
-    lds_sweeps<NC, ST, 2>(G, B, a, sp, G.coup, G.clsdat, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
-                          beta, h, key0, key1);
+  PartInfo pi;
+  if (w.nparts > 1) {
+    pi.part = w.part;
+    pi.np = w.nparts;
+    pi.hx = a.halo + w.hoff;
+    pi.halo = a.halo;
+  }
+  double dE, dM[3];
+  FrameCtx C{&G, wide ? G.coup : s_coup, wide ? G.clsdat : s_clsl, s_red, s_frm + g * FRM, g, lt, TPR};
+  if constexpr (MODE == 0)
+    lds_sweeps<NC, ST, 0, FULL>(G, B, a, sp, s_coup, s_clsl, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
+                                beta, h, key0, key1, pi, dE, dM);
+  else if constexpr (MODE == 1)
+    lds_sweeps<NC, ST, 1, FULL>(G, B, a, sp, s_coup, s_clsl, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
+                                beta, h, key0, key1, pi, dE, dM);
+  else
+    lds_sweeps<NC, ST, 2, FULL>(G, B, a, sp, G.coup, G.clsdat, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR,
+                                row, beta, h, key0, key1, pi, dE, dM);
+  __syncthreads();
+  if (pi.np == 1) {
+    stage<ST, false>(my, gsp, NC * n, lt, TPR);
+    if (lt < MCG_NSTATE) B.state[(size_t)rep * MCG_NSTATE + lt] = st[lt];
+    return;
+  }
+  // split replica (one workgroup per part, R = 1): write back own sites,
+  // publish the partial sums, count in; the last part to arrive finishes
+  // the sweep.  It needs the other parts' sites only when this sweep is
+  // measured with observables taken from the spins, or recomputed; then the
+  // parts make their sites visible device-wide (one L2 write-back per
+  // workgroup) and the last part reads them after an acquire.
+  constexpr int GD = SitesPerDraw<NC>::G;
+  const int64_t k = a.k_begin;
+  const bool meas = k >= a.nthermal && ((k - a.nthermal + 1) % a.spf) == 0;
+  const bool therm_end = (k + 1 == a.nthermal);
+  const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
+  const bool need_all = rec || (FULL && meas && (B.meas & MEAS_SPIN_OBS));
+  uint32_t* ctr = a.sync + (size_t)w.slot * SYNC_STRIDE;
+  double* xp = a.xpart + (size_t)w.slot * MAX_PARTS * 4;
+  part_copy<ST, true>(my, gsp, n, NC, s_colptr, G.ncol, GD, pi.part, pi.np, lt, TPR);
+  double v[RK] = {dE, dM[0], dM[1], dM[2], 0.0, 0.0, 0.0, 0.0};
+  group_sum8(v, s_red, g, TPR);
+  if (lt == 0)
+    for (int q = 0; q < 4; ++q) dev_store(xp + pi.part * 4 + q, v[q]);
+  stores_done();
+  __syncthreads();
+  int* s_last = (int*)s_red;
+  if (lt == 0) {
+    if (need_all) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    *s_last = (__hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) % pi.np) ==
+              (uint32_t)(pi.np - 1);
   }
   __syncthreads();
-  stage<ST, false>(my, gsp, NC * n, lt, TPR);
+  if (!*s_last) return;
+  if (need_all) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for (int p = 0; p < pi.np; ++p)
+      if (p != pi.part) part_copy<ST, false>(my, gsp, n, NC, s_colptr, G.ncol, GD, p, pi.np, lt, TPR);
+  }
+  double tE = 0.0, tM[3] = {0.0, 0.0, 0.0};
+  if (lt == 0)
+    for (int p = 0; p < pi.np; ++p) {
+      tE += p == pi.part ? v[0] : dev_load(xp + p * 4);
+      for (int q = 0; q < 3; ++q) tM[q] += p == pi.part ? v[1 + q] : dev_load(xp + p * 4 + 1 + q);
+    }
+  __syncthreads();
+  lds_sweep_tail<NC, ST, FULL>(sp, G, B, C.coup, C.clsdat, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end,
+                               rec, tE,
+                         tM, key0, key1);
   if (lt < MCG_NSTATE) B.state[(size_t)rep * MCG_NSTATE + lt] = st[lt];
 }
 
@@ -995,14 +1205,12 @@ __global__ __launch_bounds__(256) void k_measure(mcg_batch B, const int32_t* __r
 // k_pt: tempering swaps, one thread per ladder.  Pairs (t, t+1) with
 // t = parity, parity+2, ...; parity alternates with the swap count.
 // ---------------------------------------------------------------------------
-__global__ void k_pt(mcg_batch B, const int32_t* __restrict__ ladder_row0, const int32_t* __restrict__ ladder_n,
-                     int nladder, int64_t swap_index) {
-  const int L = blockIdx.x;
-  if (L >= nladder) return;
-  const int r0 = ladder_row0[L], nr = ladder_n[L];
+// Replica-exchange swaps of one ladder (rows r0 .. r0+nr-1): pairs of one
+// parity, alternating with the swap index; threads tid, tid+nthreads, ...
+// take disjoint pairs.  Accept with min(1, exp((b_a - b_b)(E_a - E_b))).
+__device__ void pt_ladder(const mcg_batch& B, int L, int r0, int nr, int64_t swap_index, int tid, int nthreads) {
   const int parity = (int)(swap_index & 1);
-  // the pairs of one parity are disjoint: one thread each
-  for (int t = parity + 2 * (int)threadIdx.x; t + 1 < nr; t += 2 * (int)blockDim.x) {
+  for (int t = parity + 2 * tid; t + 1 < nr; t += 2 * nthreads) {
     const int ra = B.who[r0 + t], rb = B.who[r0 + t + 1];
     const double Ea = B.state[(size_t)ra * MCG_NSTATE + MCG_S_E], Eb = B.state[(size_t)rb * MCG_NSTATE + MCG_S_E];
     const double ba = B.row_beta[r0 + t], bb = B.row_beta[r0 + t + 1];
@@ -1022,6 +1230,13 @@ __global__ void k_pt(mcg_batch B, const int32_t* __restrict__ ladder_row0, const
       B.pt_acc[r0 + t] += 1;
     }
   }
+}
+
+__global__ void k_pt(mcg_batch B, const int32_t* __restrict__ ladder_row0, const int32_t* __restrict__ ladder_n,
+                     int nladder, int64_t swap_index) {
+  const int L = blockIdx.x;
+  if (L >= nladder) return;
+  pt_ladder(B, L, ladder_row0[L], ladder_n[L], swap_index, (int)threadIdx.x, (int)blockDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1268,10 +1483,13 @@ int check_batch(const mcg_batch* B) {
 
 // launch-time resources kept for one call
 struct RunCtx {
-  DevBuf items, item_reps, all_reps, stream_reps, lad_row0, lad_n;
+  DevBuf items, item_reps, all_reps, stream_reps, lad_row0, lad_n, sync, xpart, halo;
   std::vector<WorkItem> hitems;
-  std::vector<int32_t> hitem_reps, hstream, hall;
+  std::vector<int32_t> hitem_reps, hstream, hall, hhalo;
   int block = 0;
+  int ncu = 0;
+  int nslots = 0;  // split replicas (arrival counters)
+  int mode_begin[4] = {0, 0, 0, 0};  // work items of link mode m: [mode_begin[m], mode_begin[m+1])
   size_t lds_max = 0;
   std::vector<int32_t> hl_row0, hl_n;
 };
@@ -1323,22 +1541,106 @@ int device_lds_max(size_t* out) {
   return 0;
 }
 
-template <int NC, typename ST>
+// link mode of a graph on the LDS path (the MODE of k_lds)
+int lds_mode(const mcg_graph& G) {
+  const bool wide = G.lcid != nullptr || G.ncls > 256;
+  return wide ? 2 : G.npat > 0 ? 0 : 1;
+}
+
+template <int NC, typename ST, int MODE, bool FULL>
 int set_lds_attr(size_t bytes) {
-  hipError_t e = hipFuncSetAttribute((const void*)k_lds<NC, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  hipError_t e = hipFuncSetAttribute((const void*)k_lds<NC, ST, MODE, FULL>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   (void)e;
   (void)hipGetLastError();
   return 0;
 }
 
-// plan the LDS work items: replicas grouped by graph, R per block
+// Extra-group tables of a replica split into np parts (see the comment on
+// split replicas at the LDS kernel): for each part, the other parts' groups
+// of each colour that its own sites depend on, closed under "neighbour of
+// lower colour".  Appends one descriptor per part (2*ncol ints: start,
+// count) and the lists to `halo`; returns the descriptors' offsets, or an
+// empty vector when `limit` is set and a part would recompute more than a
+// quarter of its own groups (then the replica is not worth splitting).
+std::vector<int32_t> halo_tables(const mcg_graph& G, int gd, int np, bool limit, std::vector<int32_t>& halo) {
+  const int n = G.n, zmax = G.zmax, ncol = G.ncol;
+  std::vector<uint32_t> link((size_t)n * zmax);
+  std::vector<int32_t> colptr(ncol + 1);
+  if (zmax > 0 && hipMemcpy(link.data(), G.link, link.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return {};
+  if (hipMemcpy(colptr.data(), G.col_ptr, colptr.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return {};
+  std::vector<int32_t> col(n), grp(n), own(n);
+  for (int c = 0; c < ncol; ++c) {
+    const int b = colptr[c], cnt = colptr[c + 1] - b, ngrp = (cnt + gd - 1) / gd;
+    for (int q = 0, part = 0; q < ngrp; ++q) {
+      while (part + 1 < np && q >= (int)((int64_t)ngrp * (part + 1) / np)) ++part;
+      for (int e = b + q * gd; e < std::min(b + (q + 1) * gd, b + cnt); ++e) {
+        col[e] = c;
+        grp[e] = q;
+        own[e] = part;
+      }
+    }
+  }
+  std::vector<int32_t> offs;
+  std::vector<char> need(n);
+  for (int part = 0; part < np; ++part) {
+    for (int e = 0; e < n; ++e) need[e] = own[e] == part;
+    for (int c = ncol - 1; c >= 0; --c)
+      for (int e = colptr[c]; e < colptr[c + 1]; ++e) {
+        if (!need[e]) continue;
+        for (int l = 0; l < zmax; ++l) {
+          const int y = (int)(link[(size_t)e * zmax + l] & 0xFFFFFFu);
+          if (y != e && col[y] < c) need[y] = 1;
+        }
+      }
+    const size_t d = halo.size();
+    offs.push_back((int32_t)d);
+    halo.resize(d + 2 * (size_t)ncol);
+    int64_t own_groups = 0, extra = 0;
+    for (int c = 0; c < ncol; ++c) {
+      const int b = colptr[c], cnt = colptr[c + 1] - b, ngrp = (cnt + gd - 1) / gd;
+      own_groups += (int64_t)(ngrp * (part + 1) / np) - (int64_t)ngrp * part / np;
+      const size_t start = halo.size();
+      int last = -1;
+      for (int e = b; e < b + cnt; ++e)
+        if (need[e] && own[e] != part && grp[e] != last) {
+          last = grp[e];
+          halo.push_back(last);
+        }
+      halo[d + 2 * c] = (int32_t)start;
+      halo[d + 2 * c + 1] = (int32_t)(halo.size() - start);
+      extra += (int64_t)(halo.size() - start);
+    }
+    if (limit && extra * 4 > own_groups) return {};
+  }
+  return offs;
+}
+
+// Plan the LDS work items.
+//
+// Replicas of one graph are packed R to a workgroup (R divides the wave
+// count, so each replica gets whole waves).  R is bounded by the LDS budget
+// and by a work target t (site updates per workgroup per sweep): t starts at
+// the batch's updates per sweep over the CU count, so that the slowest
+// workgroup carries about one CU's share.  A replica of n >= 8192 sites
+// larger than t is split over P = min(4, ceil(n/t), n/4096) workgroups
+// (parts); a launch with parts does one sweep.  The plan keeps at most one
+// workgroup per CU (t grows by 5 % until it does), because the LDS the
+// largest item asks for is given to every workgroup.  P->parts = 1 disables
+// splitting, 2..4 splits every LDS replica into that many parts.  Parts of a
+// replica are placed eight workgroup indices apart: workgroups are dealt to
+// the eight XCDs in turn, so the parts share an XCD and its L2.
 int plan_lds(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool force_stream) {
   const int ss = store_size(B->store);
+  const int gd = B->nc == 1 ? 4 : B->nc == 2 ? 2 : 1;  // sites per Philox draw
   X.hitems.clear();
   X.hitem_reps.clear();
   X.hstream.clear();
   X.hall.clear();
+  X.hhalo.clear();
+  X.nslots = 0;
   for (int r = 0; r < B->nrep; ++r) X.hall.push_back(r);
+  if (P->parts < 0 || P->parts > MAX_PARTS) return fail("parts must be in 0..%d", MAX_PARTS);
   int block = P->block;
   if (block <= 0) {
     int maxcol = 0;
@@ -1351,15 +1653,15 @@ int plan_lds(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool force_stream
   if (block % 64 || block < 64 || block > 1024) return fail("block must be a multiple of 64 in [64,1024]");
   X.block = block;
   const int nwaves = block / 64;
-  // group replicas by graph, preserving order
+  // group replicas by graph, preserving order; Rmax from the LDS budget
   std::vector<std::vector<int>> by(B->ngraph);
   for (int r = 0; r < B->nrep; ++r) by[B->reps_host[r].graph].push_back(r);
+  std::vector<int> rmax(B->ngraph, 0);
+  double work = 0;
   for (int gi = 0; gi < B->ngraph; ++gi) {
     const mcg_graph& G = B->graphs_host[gi];
-    const std::vector<int>& rs = by[gi];
-    if (rs.empty()) continue;
+    if (by[gi].empty()) continue;
     const size_t per = lds_replica_bytes(B->nc, G.n, ss);
-    int Rmax = 0;
     if (!force_stream) {
       for (int R = nwaves; R >= 1; --R) {
         if (nwaves % R) continue;
@@ -1367,41 +1669,140 @@ int plan_lds(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool force_stream
         const bool wide = G.lcid != nullptr || G.ncls > 256;
         if (lds_fixed_bytes(wide ? 0 : G.ncoup, wide ? 0 : G.ncls, G.ncol, G.npat, G.zmax, G.n, nwaves, R) + R * per <=
             X.lds_max) {
-          Rmax = R;
+          rmax[gi] = R;
           break;
         }
       }
     }
-    if (Rmax == 0) {
-      for (int r : rs) X.hstream.push_back(r);
-      continue;
+    if (rmax[gi] == 0)
+      for (int r : by[gi]) X.hstream.push_back(r);
+    else
+      work += (double)G.n * by[gi].size();
+  }
+  if (work == 0) return 0;
+  const int ncu = std::max(1, X.ncu);
+  std::vector<char> nosplit(B->ngraph, 0);
+  struct Split {
+    int gi, rep, np;
+  };
+  for (int attempt = 0; attempt <= B->ngraph; ++attempt) {
+    double t = std::max(1.0, std::ceil(work / ncu));
+    std::vector<WorkItem> items;
+    std::vector<int32_t> ireps;
+    std::vector<Split> splits;
+    std::vector<int> gnp(B->ngraph, 1);
+    for (int iter = 0; iter < 400; ++iter) {
+      items.clear();
+      ireps.clear();
+      splits.clear();
+      size_t nsplit_blocks = 0;
+      for (int gi = 0; gi < B->ngraph; ++gi) {
+        gnp[gi] = 1;
+        if (rmax[gi] == 0) continue;
+        const mcg_graph& G = B->graphs_host[gi];
+        const std::vector<int>& rs = by[gi];
+        int np = 1;
+        if (P->parts >= 2)
+          np = P->parts;
+        else if (P->parts == 0 && !nosplit[gi] && G.n >= 8192 && G.n > t)
+          np = std::min({MAX_PARTS, (int)std::ceil(G.n / t), G.n / 4096});
+        if (np >= 2) {
+          gnp[gi] = np;
+          for (int r : rs) splits.push_back(Split{gi, r, np});
+          nsplit_blocks += (size_t)np * rs.size();
+          continue;
+        }
+        int R = rmax[gi];
+        if (P->reps_per_block <= 0) R = std::min(R, std::max(1, (int)(t / G.n)));
+        while (nwaves % R) --R;
+        for (size_t i = 0; i < rs.size(); i += R) {
+          const int cnt = (int)std::min<size_t>(R, rs.size() - i);
+          // the tail item takes the largest divisor of nwaves that fits, repeatedly
+          for (int q = 0; q < cnt;) {
+            int Rq = cnt - q;
+            while (nwaves % Rq) --Rq;
+            items.push_back(WorkItem{gi, Rq, 0, 0, 1, 0, 0});
+            for (int k = 0; k < Rq; ++k) ireps.push_back(rs[i + q + k]);
+            q += Rq;
+          }
+        }
+      }
+      if (P->parts == 0 && !splits.empty() && items.size() + nsplit_blocks > (size_t)ncu) {
+        t = t * 1.05 + 1.0;
+        continue;
+      }
+      break;
     }
-    const int R = Rmax;
-    for (size_t i = 0; i < rs.size(); i += R) {
-      const int cnt = (int)std::min<size_t>(R, rs.size() - i);
-      // a block's groups must tile the block: pad by repeating is not allowed,
-      // so shrink R for the tail item to a divisor of nwaves
-      int Rt = cnt;
-      while (nwaves % Rt) --Rt;
-      for (size_t q = 0; q < (size_t)cnt; q += Rt) {
-        const int c2 = (int)std::min<size_t>(Rt, cnt - q);
-        int Rq = c2;
-        while (nwaves % Rq) --Rq;
-        // c2 == Rq always holds here because Rt divides nwaves and c2 <= Rt
-        X.hitems.push_back(WorkItem{gi, Rq, (int)X.hitem_reps.size(), 0});
-        for (int k = 0; k < Rq; ++k) X.hitem_reps.push_back(rs[i + q + k]);
-        if (Rq < c2) return fail("internal: work item split");
+    if (splits.size() > 65535) return fail("too many split replicas");
+    // extra-group tables, one set per split graph
+    std::vector<std::vector<int32_t>> hoffs(B->ngraph);
+    bool redo = false;
+    X.hhalo.clear();
+    for (int gi = 0; gi < B->ngraph; ++gi) {
+      if (gnp[gi] < 2) continue;
+      hoffs[gi] = halo_tables(B->graphs_host[gi], gd, gnp[gi], P->parts == 0, X.hhalo);
+      if (hoffs[gi].empty()) {
+        if (P->parts >= 2) return fail("graph %d: reading the link table for the split failed", gi);
+        nosplit[gi] = 1;
+        redo = true;
       }
     }
+    if (redo) continue;
+    // place parts: in each run of up to 8 split replicas with np parts,
+    // part p of the run's replica j goes to index base + j + 8 p
+    std::vector<WorkItem> out;
+    std::vector<int32_t> oreps;
+    X.nslots = 0;
+    for (size_t j = 0; j < splits.size();) {
+      const int np = splits[j].np;
+      size_t e = j;
+      while (e < splits.size() && e - j < 8 && splits[e].np == np) ++e;
+      std::vector<WorkItem> run((size_t)8 * np, WorkItem{-1, 0, 0, 0, 0, 0, 0});
+      std::vector<int32_t> run_rep((size_t)8 * np, -1);
+      for (size_t q = j; q < e; ++q) {
+        const uint16_t slot = (uint16_t)X.nslots++;
+        const int gi = splits[q].gi;
+        for (int p = 0; p < np; ++p) {
+          run[(q - j) + (size_t)8 * p] = WorkItem{gi, 1, 0, (uint8_t)p, (uint8_t)np, slot, hoffs[gi][p]};
+          run_rep[(q - j) + (size_t)8 * p] = splits[q].rep;
+        }
+      }
+      for (size_t q = 0; q < run.size(); ++q) {
+        if (run[q].graph < 0) continue;  // holes of a short run
+        run[q].off = (int)oreps.size();
+        oreps.push_back(run_rep[q]);
+        out.push_back(run[q]);
+      }
+      j = e;
+    }
+    size_t ir = 0;
+    for (WorkItem w : items) {
+      w.off = (int)oreps.size();
+      for (int k = 0; k < w.nrep; ++k) oreps.push_back(ireps[ir + k]);
+      ir += w.nrep;
+      out.push_back(w);
+    }
+    // one kernel per link mode: sort the items by mode (stable, so the
+    // parts of a split replica stay eight indices apart)
+    std::stable_sort(out.begin(), out.end(), [&](const WorkItem& x, const WorkItem& y) {
+      return lds_mode(B->graphs_host[x.graph]) < lds_mode(B->graphs_host[y.graph]);
+    });
+    for (int m = 0; m <= 3; ++m) X.mode_begin[m] = 0;
+    for (const WorkItem& w : out) X.mode_begin[lds_mode(B->graphs_host[w.graph]) + 1]++;
+    for (int m = 1; m <= 3; ++m) X.mode_begin[m] += X.mode_begin[m - 1];
+    X.hitems = out;
+    X.hitem_reps = oreps;
+    return 0;
   }
-  return 0;
+  return fail("internal: LDS plan did not converge");
 }
 
 template <int NC, typename ST>
-size_t lds_bytes_for(const mcg_batch* B, const RunCtx& X) {
+size_t lds_bytes_for(const mcg_batch* B, const RunCtx& X, int i0, int i1) {
   size_t mx = 0;
   const int nwaves = X.block / 64;
-  for (const WorkItem& w : X.hitems) {
+  for (int i = i0; i < i1; ++i) {
+    const WorkItem& w = X.hitems[i];
     const mcg_graph& G = B->graphs_host[w.graph];
     const bool wide = G.lcid != nullptr || G.ncls > 256;
     const size_t b = lds_fixed_bytes(wide ? 0 : G.ncoup, wide ? 0 : G.ncls, G.ncol, G.npat, G.zmax, G.n, nwaves,
@@ -1412,15 +1813,39 @@ size_t lds_bytes_for(const mcg_batch* B, const RunCtx& X) {
   return mx;
 }
 
-template <int NC, typename ST>
-int launch_lds(const mcg_batch* B, RunCtx& X, const SweepArgs& a, hipStream_t s, hipEvent_t e0 = nullptr,
-               hipEvent_t e1 = nullptr) {
-  if (X.hitems.empty()) return 0;
-  const size_t bytes = lds_bytes_for<NC, ST>(B, X);
-  set_lds_attr<NC, ST>(bytes);
-  hipExtLaunchKernelGGL((k_lds<NC, ST>), dim3((unsigned)X.hitems.size()), dim3(X.block), (uint32_t)bytes, s, e0,
-                        e1, 0, *B, (const WorkItem*)X.items.p, (const int32_t*)X.item_reps.p, a);
+template <int NC, typename ST, int MODE, bool FULL>
+int launch_lds_range(const mcg_batch* B, RunCtx& X, const SweepArgs& a, hipStream_t s, int i0, int i1, hipEvent_t e0,
+                     hipEvent_t e1) {
+  const size_t bytes = lds_bytes_for<NC, ST>(B, X, i0, i1);
+  set_lds_attr<NC, ST, MODE, FULL>(bytes);
+  hipExtLaunchKernelGGL((k_lds<NC, ST, MODE, FULL>), dim3((unsigned)(i1 - i0)), dim3(X.block), (uint32_t)bytes, s,
+                        e0, e1, 0, *B, (const WorkItem*)X.items.p + i0, (const ItemRep*)X.item_reps.p, a);
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// one launch per link mode present (work items are sorted by mode)
+template <int NC, typename ST>
+int launch_lds(const mcg_batch* B, RunCtx& X, const SweepArgs& a, hipStream_t s, LaunchTimer& T) {
+  const bool full = (B->meas & MEAS_SPIN_OBS) != 0;
+  for (int m = 0; m < 3; ++m) {
+    const int i0 = X.mode_begin[m], i1 = X.mode_begin[m + 1];
+    if (i1 <= i0) continue;
+    hipEvent_t e0, e1;
+    if (T.pair(&e0, &e1)) return -1;
+    int r;
+    if (m == 0)
+      r = full ? launch_lds_range<NC, ST, 0, true>(B, X, a, s, i0, i1, e0, e1)
+               : launch_lds_range<NC, ST, 0, false>(B, X, a, s, i0, i1, e0, e1);
+    else if (m == 1)
+      r = full ? launch_lds_range<NC, ST, 1, true>(B, X, a, s, i0, i1, e0, e1)
+               : launch_lds_range<NC, ST, 1, false>(B, X, a, s, i0, i1, e0, e1);
+    else
+      r = full ? launch_lds_range<NC, ST, 2, true>(B, X, a, s, i0, i1, e0, e1)
+               : launch_lds_range<NC, ST, 2, false>(B, X, a, s, i0, i1, e0, e1);
+    if (r) return -1;
+    T.launches += 1;
+  }
   return 0;
 }
 
@@ -1519,13 +1944,35 @@ template <int NC, typename ST>
 int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
   RunCtx X;
   if (device_lds_max(&X.lds_max)) return -1;
+  {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&X.ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  }
   const bool force_stream = P->kernel == 2 || P->kernel == 3;
   const bool plain = P->kernel == 3;
   if (plan_lds(B, P, X, force_stream)) return -1;
+  if (X.nslots > 0) {
+    const int64_t total_sweeps = (int64_t)P->nthermal + (int64_t)P->nsweep * std::max(1, P->sweeps_per_frame);
+    if (total_sweeps * MAX_PARTS >= ((int64_t)1 << 32))
+      return fail("too many sweeps in one call for split replicas (%lld); split the call", (long long)total_sweeps);
+    std::vector<uint32_t> zero((size_t)X.nslots * SYNC_STRIDE, 0u);
+    if (X.sync.upload(zero.data(), zero.size() * 4, s)) return -1;
+    std::vector<double> zx((size_t)X.nslots * MAX_PARTS * 4, 0.0);
+    if (X.xpart.upload(zx.data(), zx.size() * 8, s)) return -1;
+    if (X.halo.upload(X.hhalo.data(), X.hhalo.size() * 4, s)) return -1;
+  }
   if (P->kernel == 1 && !X.hstream.empty())
     return fail("LDS-resident kernel requested but %zu replica(s) do not fit in LDS", X.hstream.size());
   if (X.items.upload(X.hitems.data(), X.hitems.size() * sizeof(WorkItem), s)) return -1;
-  if (X.item_reps.upload(X.hitem_reps.data(), X.hitem_reps.size() * 4, s)) return -1;
+  {
+    std::vector<ItemRep> recs(X.hitem_reps.size());
+    for (size_t i = 0; i < recs.size(); ++i) {
+      const mcg_replica& R = B->reps_host[X.hitem_reps[i]];
+      recs[i] = ItemRep{R.spin_off, X.hitem_reps[i], (int32_t)R.serial, R.row0, 0};
+    }
+    if (X.item_reps.upload(recs.data(), recs.size() * sizeof(ItemRep), s)) return -1;
+  }
   if (X.stream_reps.upload(X.hstream.data(), X.hstream.size() * 4, s)) return -1;
   if (X.all_reps.upload(X.hall.data(), X.hall.size() * 4, s)) return -1;
   // ladders
@@ -1578,15 +2025,16 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
   LaunchTimer T;
   T.on = P->timing != nullptr;
   double lds_updates_per_sweep = 0;
-  for (const WorkItem& w : X.hitems) lds_updates_per_sweep += (double)w.nrep * B->graphs_host[w.graph].n;
-  const int64_t chunk = P->pt_every > 0 ? P->pt_every : total;
+  for (const WorkItem& w : X.hitems)
+    if (w.part == 0) lds_updates_per_sweep += (double)w.nrep * B->graphs_host[w.graph].n;
+  // split replicas finish every launch: one sweep per launch
+  const int64_t chunk = X.nslots > 0 ? 1 : P->pt_every > 0 ? P->pt_every : total;
   for (int64_t k = 0; k < total;) {
     const int64_t ns = std::min<int64_t>(chunk, total - k);
-    SweepArgs a{k, P->sweep0, (int32_t)ns, (int32_t)nth, spf, P->frame0};
+    SweepArgs a{k, P->sweep0, (int32_t)ns, (int32_t)nth, spf, P->frame0, (uint32_t*)X.sync.p, (double*)X.xpart.p,
+                (const int32_t*)X.halo.p};
     if (!X.hitems.empty()) {
-      hipEvent_t e0, e1;
-      if (T.pair(&e0, &e1) || launch_lds<NC, ST>(B, X, a, s, e0, e1)) return -1;
-      T.launches += 1;
+      if (launch_lds<NC, ST>(B, X, a, s, T)) return -1;
       T.sweeps += (double)ns;
       T.updates += lds_updates_per_sweep * (double)ns;
     }

@@ -152,7 +152,7 @@ def build(name: str, store: str = "f32", nbins: int = 32, nframes: int = 1, meas
     else:
         raise ValueError(name)
     temps = np.asarray(temps)
-    beta = 1.0 / temps
+    beta = 1.0 / np.maximum(temps, 0.1)  # the reference runs T < 0.1 at 0.1 (mcMain.py:21)
     b = Batch(nc, graphs, reps, beta, fields, inits, store=("i8" if nc == 1 else store), meas=meas, nbins=nbins,
               nframes=nframes, seed=SEED, wolff=wolff, device=device)
     return Workload(name, nc, b, lattices, temps, np.asarray(fields), np.asarray(rgraph),

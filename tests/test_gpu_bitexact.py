@@ -204,3 +204,90 @@ def test_lds_without_link_patterns(gpu):
     ref = _cpu_spins(3, init, "f32")
     mcref.cb_sweeps(3, 0, nbr, J, S, D, g.colour, betas, [0.2, 0.2], np.arange(2, dtype=np.uint32), SEED, 0, 6, ref)
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+# ---------------------------------------------------------------------------
+# replicas split over several workgroups (parts exchange each colour class)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("parts", [2, 4])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_split_bitexact(gpu, case, parts):
+    """A replica split over 2 or 4 workgroups follows the checker's
+    trajectory bit for bit (the Philox groups are dealt out, not changed)."""
+    nc, mk, betas, h = CASES[case]
+    Lt = mk()
+    print("case", case, "parts", parts, "n", Lt.n, flush=True)
+    b, g, init = _mk_batch(nc, Lt, betas, h)
+    nsw = 12
+    b.sweeps(0, nsw, kernel=L.KERNEL_LDS, parts=parts)
+    got = _gpu_spins(b, len(betas))
+    ref = _cpu_spins(nc, init, "f32")
+    nbr, J, S, D = _lattice_tables(Lt, nc)
+    mcref.cb_sweeps(nc, 0 if nc > 1 else 2, nbr, J, S, D, g.colour, betas, [h] * len(betas),
+                    np.arange(len(betas), dtype=np.uint32), SEED, 0, nsw, ref)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), "spins differ"
+    st = b.state_data()
+    for k in range(len(betas)):
+        e_cpu = mcref.cb_energy(nc, nbr, J, S, D, h, ref[k])
+        assert abs(st[k, L.S_E] - e_cpu[0]) <= 1e-3 * max(1.0, abs(e_cpu[0])), (k, st[k, L.S_E], e_cpu[0])
+        np.testing.assert_allclose(st[k, L.S_M:L.S_M + 3], e_cpu[1:], rtol=1e-4, atol=1e-3)
+    assert b.error_flag() == 0
+
+
+def _ladder_batch(parts_seed=0):
+    """Two tempering ladders (XY L=16 and L=24, 8 rungs each) with every
+    measurement on."""
+    graphs, reps, betas, spins = [], [], [], []
+    rng = np.random.default_rng(11)
+    for gi, L_ in enumerate((16, 24)):
+        Lt = lat.square(L_, J=-1.0, D=(0.05, 0.0, 0.0))
+        graphs.append(compile_graph(2, Lt.nbr, Lt.J, Lt.S, Lt.D, pairs=Lt.corr_pairs, rOrb=Lt.rOrb,
+                                    rOrbCluster=Lt.rOrbCluster, rlink=Lt.rlink))
+        for r in range(8):
+            reps.append(ReplicaSpec(gi, gi * 8, nrung=8, ladder=gi, rung=r))
+            betas.append(1 / (0.7 + 0.07 * r))
+            v = rng.normal(size=(Lt.n, 2))
+            spins.append(v / np.linalg.norm(v, axis=1, keepdims=True))
+    return graphs, reps, betas, spins
+
+
+@pytest.mark.parametrize("parts", [2, 3])
+def test_split_run_matches_unsplit(gpu, parts):
+    """Measured run with tempering: split and unsplit replicas give the same
+    spins, swaps and accumulators (up to FP64 summation order)."""
+    out = []
+    for pp in (1, parts):
+        graphs, reps, betas, spins = _ladder_batch()
+        b = Batch(2, graphs, reps, betas, [0.02] * len(betas), spins, meas=L.MEAS_PAIRS | L.MEAS_RNORM, nbins=4,
+                  nframes=40, recompute_every=7)
+        b.init()
+        b.run(10, 40, 1, pt_every=1, kernel=L.KERNEL_LDS, parts=pp)
+        out.append((np.concatenate([b.replica_spins(k).ravel() for k in range(len(reps))]), b.accumulators(),
+                    b.bin_data(),
+                    b.pt_acc.cpu().numpy(), b.rung.cpu().numpy(), b.state_data(), b.error_flag()))
+    (s1, a1, bn1, p1, r1, st1, e1), (s2, a2, bn2, p2, r2, st2, e2) = out
+    assert e1 == 0 and e2 == 0
+    assert np.array_equal(s1.view(np.uint32), s2.view(np.uint32))
+    assert np.array_equal(p1, p2) and np.array_equal(r1, r2)
+    np.testing.assert_allclose(a2, a1, rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(bn2, bn1, rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(st2[:, :4], st1[:, :4], rtol=1e-9, atol=1e-9)
+
+
+def test_auto_split_large_replicas(gpu):
+    """Auto planning splits replicas larger than one CU's share (here two
+    L=128 XY replicas, 4 parts each) and keeps the trajectory."""
+    nc, Lt = 2, lat.square(128, J=-1.0)
+    betas = [0.9, 1.4]
+    res = []
+    for parts in (1, 0):
+        b, g, init = _mk_batch(nc, Lt, betas, 0.0)
+        b.sweeps(0, 5, kernel=L.KERNEL_LDS, parts=parts)
+        res.append(_gpu_spins(b, 2))
+        assert b.error_flag() == 0
+    ref = _cpu_spins(nc, init, "f32")
+    nbr, J, S, D = _lattice_tables(Lt, nc)
+    mcref.cb_sweeps(nc, 0, nbr, J, S, D, g.colour, betas, [0.0, 0.0], np.arange(2, dtype=np.uint32), SEED, 0, 5, ref)
+    assert np.array_equal(res[0].view(np.uint32), ref.view(np.uint32))
+    assert np.array_equal(res[1].view(np.uint32), ref.view(np.uint32))
