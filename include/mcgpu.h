@@ -156,7 +156,8 @@ typedef struct mcg_plan {
                               8192+ sites that exceed one CU's share of the batch; 1 = never
                               split; 2..4 = split every LDS replica).  A launch with split
                               replicas runs one sweep. */
-  int32_t pad_;
+  int32_t no_persist;      /* nonzero: never use persistent launches of the LDS kernel (see
+                              mcg_run); 0: use them when every workgroup can be resident */
   int64_t sweep0;          /* global sweep index of the first sweep (Philox counter) */
   double *timing;          /* host double[4] or NULL: when set, mcg_run brackets every
                               launch of the sweep kernel with HIP events on the stream

@@ -50,7 +50,7 @@ class Batch(C.Structure):
 class Plan(C.Structure):
     _fields_ = [(k, C.c_int32) for k in ("algorithm", "nthermal", "nsweep", "sweeps_per_frame", "pt_every", "kernel",
                                          "block", "reps_per_block", "clusters_per_sweep", "frame0", "parts",
-                                         "pad_")] + \
+                                         "no_persist")] + \
                [("sweep0", C.c_int64), ("timing", C.POINTER(C.c_double))]
 
 

@@ -344,19 +344,52 @@ __device__ __forceinline__ float site_energy(const SV& sp, int i, const uint32_t
 // ---------------------------------------------------------------------------
 // group reductions (a group = TPR consecutive threads, TPR a multiple of 64)
 // ---------------------------------------------------------------------------
+// accesses at the device coherence point (device-scope atomics): visible to
+// every workgroup of the launch whatever XCD it runs on
+template <typename T>
+__device__ __forceinline__ void dev_store(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T dev_load(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void dev_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+
 static constexpr int RK = 8;  // values per reduction call
 static constexpr int FRM_ACC = 96;  // frame scratch: accumulator staging starts here
 
+// FP64 value moved across lanes by DPP (two 32-bit halves); lanes of rows
+// outside ROWMASK get 0
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const int lo = __double2loint(x), hi = __double2hiint(x);
+  const int l2 = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROWMASK, 0xF, false);
+  const int h2 = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROWMASK, 0xF, false);
+  return __hiloint2double(h2, l2);
+}
+
+// sum over the 64 lanes of a wave (every lane active); the total lands in
+// lane 63.  Quad swaps, half-row and row mirrors give every lane its 16-lane
+// row sum; row broadcasts 15 and 31 then add the rows up.  DPP moves are
+// VALU operations: no LDS traffic, unlike a shuffle butterfly.
+__device__ __forceinline__ double wave_sum_f64(double x) {
+  x += dpp_f64<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
+  x += dpp_f64<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]
+  x += dpp_f64<0x141, 0xF>(x);  // row_half_mirror
+  x += dpp_f64<0x140, 0xF>(x);  // row_mirror
+  x += dpp_f64<0x142, 0xA>(x);  // row_bcast:15 into rows 1, 3
+  x += dpp_f64<0x143, 0xC>(x);  // row_bcast:31 into rows 2, 3
+  return x;
+}
+
+// sum of RK values over the TPR threads of group g (TPR a multiple of 64);
+// the result is valid in the group's first thread.  All threads call it.
 __device__ __forceinline__ void group_sum8(double* v, double* s_red, int g, int TPR) {
 #pragma unroll
-  for (int k = 0; k < RK; ++k) {
-    double x = v[k];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    v[k] = x;
-  }
+  for (int k = 0; k < RK; ++k) v[k] = wave_sum_f64(v[k]);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0)
+  if (lane == 63)
     for (int k = 0; k < RK; ++k) s_red[wave * RK + k] = v[k];
   __syncthreads();
   if ((int)threadIdx.x == g * TPR) {
@@ -581,19 +614,20 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
     for (int q = 0; q < MCG_NACC + MCG_NBINQ + 1; ++q) Cn[q] = 0.0;
     const double n = G.n;
     const double nLat = G.nlat > 0 ? G.nlat : G.n;
+    const double inv_nLat = 1.0 / nLat;
     const bool pairs = (flags & MCG_MEAS_PAIRS) && G.nlat > 0;
     double E = st[MCG_S_E];
     if (flags & MCG_ISING_RELE) E -= st[MCG_S_EREF];
-    const double e = E / n;
+    const double e = E * (1.0 / n);
     const double M0 = st[MCG_S_M], M1 = st[MCG_S_M + 1], M2 = st[MCG_S_M + 2];
     double Mf;
     const int ax = NC == 3 ? 2 : 0;
     if constexpr (NC == 1) {
-      Mf = pairs ? F[0] / nLat : M0 / nLat;
+      Mf = (pairs ? F[0] : M0) * inv_nLat;
       if (pairs) {
-        Cn[MCG_A_SI] = fabs(F[0]) / nLat;
-        Cn[MCG_A_SJ] = fabs(F[3]) / nLat;
-        Cn[MCG_A_SIJ] = F[6] / nLat;
+        Cn[MCG_A_SI] = fabs(F[0]) * inv_nLat;
+        Cn[MCG_A_SJ] = fabs(F[3]) * inv_nLat;
+        Cn[MCG_A_SIJ] = F[6] * inv_nLat;
         Cn[MCG_A_LAST] = F[7];
       }
       Cn[MCG_A_TOT] = M0;
@@ -605,22 +639,22 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
         dir[0] /= ml; dir[1] /= ml; dir[2] /= ml;
       }
       if (NC == 3 || !pairs)
-        Mf = ml / nLat;
+        Mf = ml * inv_nLat;
       else
-        Mf = sqrt(F[0] * F[0] + F[1] * F[1]) / nLat;
+        Mf = sqrt(F[0] * F[0] + F[1] * F[1]) * inv_nLat;
       if (pairs) {
         for (int k = 0; k < NC; ++k) {
-          Cn[MCG_A_SI + k] = fabs(F[k] / nLat);
-          Cn[MCG_A_SJ + k] = fabs(F[3 + k] / nLat);
+          Cn[MCG_A_SI + k] = fabs(F[k] * inv_nLat);
+          Cn[MCG_A_SJ + k] = fabs(F[3 + k] * inv_nLat);
         }
-        Cn[MCG_A_SIJ] = F[6] / nLat;
-        Cn[MCG_A_ZI] = (dir[0] * F[0] + dir[1] * F[1] + dir[2] * F[2]) / nLat;
-        Cn[MCG_A_ZJ] = (dir[0] * F[3] + dir[1] * F[4] + dir[2] * F[5]) / nLat;
-        Cn[MCG_A_HI] = F[ax] / nLat;
-        Cn[MCG_A_HJ] = F[3 + ax] / nLat;
+        Cn[MCG_A_SIJ] = F[6] * inv_nLat;
+        Cn[MCG_A_ZI] = (dir[0] * F[0] + dir[1] * F[1] + dir[2] * F[2]) * inv_nLat;
+        Cn[MCG_A_ZJ] = (dir[0] * F[3] + dir[1] * F[4] + dir[2] * F[5]) * inv_nLat;
+        Cn[MCG_A_HI] = F[ax] * inv_nLat;
+        Cn[MCG_A_HJ] = F[3 + ax] * inv_nLat;
       }
-      Cn[MCG_A_ZT] = (dir[0] * Mv[0] + dir[1] * Mv[1] + dir[2] * Mv[2]) / nLat;
-      Cn[MCG_A_HT] = Mv[ax] / nLat;
+      Cn[MCG_A_ZT] = (dir[0] * Mv[0] + dir[1] * Mv[1] + dir[2] * Mv[2]) * inv_nLat;
+      Cn[MCG_A_HT] = Mv[ax] * inv_nLat;
       if ((flags & MCG_MEAS_Q) && NC == 3 && G.ncirc > 0) Cn[MCG_A_Q] = F[8] / REF_PI / 4;
       if ((flags & MCG_MEAS_RNORM) && G.nr > 0) {
         for (int k = 0; k < NC; ++k) {
@@ -631,7 +665,7 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
       }
       // groups: the total (all sites / nLat) is group ng
       if (ngq > 0)
-        for (int k = 0; k < 3; ++k) F[32 + 3 * ngq + k] = Mv[k] / nLat;
+        for (int k = 0; k < 3; ++k) F[32 + 3 * ngq + k] = Mv[k] * inv_nLat;
     }
     if ((flags & MCG_MEAS_RNORM) && G.nr > 0) {
       const double er = (F[27] / 2 + F[28]) / G.nr;
@@ -646,9 +680,13 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
     Cn[MCG_A_MM] = Mf;  // applied as A[MM] += A[MPREV] * Mf; A[MPREV] = Mf
     Cn[MCG_A_COUNT] = 1.0;
     Cn[MCG_A_ABSM] = fabs(Mf);
-    // bins for error bars
-    int64_t nf = B.nframes > 0 ? B.nframes : 1;
-    int64_t bin = (frame * (int64_t)B.nbins) / nf;
+    // bins for error bars: bin = floor(frame * nbins / nframes), from an FP64
+    // quotient corrected to the exact integer floor
+    const int64_t nf = B.nframes > 0 ? B.nframes : 1;
+    const int64_t num = frame * (int64_t)B.nbins;
+    int64_t bin = (int64_t)floor((double)num / (double)nf);
+    if (bin * nf > num) --bin;
+    if ((bin + 1) * nf <= num) ++bin;
     if (bin >= B.nbins) bin = B.nbins - 1;
     if (bin < 0) bin = 0;
     Cn[MCG_NACC + 0] = e;
@@ -659,21 +697,24 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
     Cn[MCG_NACC + MCG_NBINQ] = (double)bin;
   }
   __syncthreads();
+  // A row's accumulators are written by whichever replica holds the row, so
+  // within one persistent launch by workgroups on different XCDs: the
+  // updates go to the device coherence point (device-scope atomics).
   double* A = B.acc + (size_t)row * MCG_NACC;
   const int nge = ngq > 0 ? (ngq + 1) * (ngq + 1) : 0;
   for (int q = lt; q < MCG_NACC + MCG_NBINQ + nge; q += TPR) {
     if (q < MCG_NACC) {
       if (q == MCG_A_MPREV) continue;
       if (q == MCG_A_MM) {
-        const double prev = A[MCG_A_MPREV];
-        A[MCG_A_MM] += prev * Cn[MCG_A_MM];
-        A[MCG_A_MPREV] = Cn[MCG_A_MM];
-      } else {
-        A[q] += Cn[q];
+        const double prev = dev_load(A + MCG_A_MPREV);
+        dev_add(A + MCG_A_MM, prev * Cn[MCG_A_MM]);
+        dev_store(A + MCG_A_MPREV, Cn[MCG_A_MM]);
+      } else if (Cn[q] != 0.0) {  // adding 0 changes nothing
+        dev_add(A + q, Cn[q]);
       }
     } else if (q < MCG_NACC + MCG_NBINQ) {
       const int64_t bin = (int64_t)Cn[MCG_NACC + MCG_NBINQ];
-      B.bins[((size_t)row * B.nbins + bin) * MCG_NBINQ + (q - MCG_NACC)] += Cn[q];
+      if (Cn[q] != 0.0) dev_add(B.bins + ((size_t)row * B.nbins + bin) * MCG_NBINQ + (q - MCG_NACC), Cn[q]);
     } else {
       // orbital groups (heisenbergLib.c:805-830): <g_a . g_b> and <(g_a . g_a)^2>
       const int j = q - MCG_NACC - MCG_NBINQ, ga = j / (ngq + 1), gb = j % (ngq + 1);
@@ -681,8 +722,8 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
       const double* va = F + 32 + 3 * ga;
       const double* vb = F + 32 + 3 * gb;
       const double d = va[0] * vb[0] + va[1] * vb[1] + va[2] * vb[2];
-      GA[j] += d;
-      if (ga == gb) GA[(ngq + 1) * (ngq + 1) + ga] += d * d;
+      dev_add(GA + j, d);
+      if (ga == gb) dev_add(GA + (ngq + 1) * (ngq + 1) + ga, d * d);
     }
   }
 }
@@ -729,6 +770,8 @@ struct ItemRep {
 static constexpr int MAX_PARTS = 4;
 static constexpr int SYNC_STRIDE = 16;  // uint32 per replica counter (one 64-byte line)
 
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
 struct SweepArgs {
   int64_t k_begin;      // run-relative index of the first sweep of this launch
   int64_t philox_base;  // added to k for the Philox counter
@@ -739,11 +782,31 @@ struct SweepArgs {
   uint32_t* sync;       // split replicas: arrival counters, zero at run start
   double* xpart;        // split replicas: per-part (dE, dM) partial sums
   const int32_t* halo;  // split replicas: extra-group tables
+  // persistent launches (many sweeps, tempering inside the kernel)
+  uint32_t* bar;             // grid barrier counter, zero at run start
+  int32_t bar_base;          // barriers passed before this launch
+  int32_t pt_every;          // tempering period in sweeps (0: none)
+  int32_t has_split;         // the launch holds split replicas
+  int32_t nladder;
+  const int32_t* lad_row0;
+  const int32_t* lad_n;
+  const int32_t* lad_pre;    // [2][nladder+1] pairs per parity, prefix sums
+  const uint8_t* rep_np;     // parts of each replica (1: whole)
+  double* eslot;             // [2][nrep][MAX_PARTS] published energies
+  double* mslot;             // [2][nrep][MAX_PARTS][3] published magnetisation changes
+  void* xbuf;                // [2][nslots][MAX_PARTS][max_export][NC] exported sites
+  int32_t max_export;
+  int32_t nslots;
 };
+
+// LDS of the tempering state a persistent launch keeps in every workgroup:
+// who and accepted-swap counts per row, rung per replica, beta and h per row
+__host__ __device__ inline size_t persist_lds_bytes(int nrow, int nrep) {
+  return align16((size_t)(2 * nrow + nrep) * 4) + align16((size_t)2 * nrow * 4);
+}
 
 static constexpr int FRM = 96 + 48;  // frame scratch doubles per group: scalars, then accumulator staging
 
-__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
 // copy one replica between HBM (16-byte aligned) and LDS in 16-byte vectors,
 // four in flight per thread
@@ -815,19 +878,36 @@ __device__ __forceinline__ void part_range(int b, int cnt, int ngrp, int gd, int
   e1 = b + min(qb * gd, cnt);
 }
 
-template <typename T>
-__device__ __forceinline__ void dev_store(T* p, T v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ T dev_load(const T* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 // every store this wave issued has been acknowledged
 __device__ __forceinline__ void stores_done() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
   __builtin_amdgcn_s_waitcnt(0);
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Grid-wide barrier of a persistent launch (every workgroup resident):
+// count in, wait until the counter reaches target (the barrier's index times
+// the workgroup count).  Bounded: on time-out bit 16 of the error flag is
+// raised and false returned, and the caller leaves the kernel.
+__device__ __forceinline__ bool grid_barrier(uint32_t* bar, uint32_t target, int32_t* err, int* s_flag) {
+  stores_done();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int ok = 1;
+    uint32_t it = 0;
+    while ((int32_t)(dev_load(bar) - target) < 0) {
+      if (++it > (1u << 20)) {
+        atomicOr(err, 16);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    *s_flag = ok;
+  }
+  __syncthreads();
+  return *s_flag != 0;
 }
 
 template <int S>
@@ -870,28 +950,23 @@ __device__ void part_copy(ST* lds_s, ST* gsp, int n, int nc, const int32_t* colp
   }
 }
 
+// ints per part in the split tables: (start, count) of the extra groups per
+// colour, then (start, count) of the export list
+__host__ __device__ inline int halo_desc_ints(int ncol) { return 2 * ncol + 2; }
+
 struct PartInfo {
   int part = 0, np = 1;            // this workgroup's part of the replica
   const int32_t* hx = nullptr;     // extra groups: [2c] start, [2c+1] count per colour, into halo
   const int32_t* halo = nullptr;
 };
 
+// end-of-sweep work after the state is current: full recompute (drift
+// check), energy reference at the end of thermalisation, measured frame
 template <int NC, typename ST, bool FULL, typename SV>
-__device__ __forceinline__ void lds_sweep_tail(const SV& sv, const mcg_graph& G, const mcg_batch& B, const float* s_coup,
-                                               const float* s_cls, double* s_red, double* st, FrameCtx& C, int g,
-                                               int lt, int TPR, int row, float h, int64_t k, const SweepArgs& a,
-                                               bool meas, bool therm_end, bool rec, double& dE, double* dM,
-                                               uint32_t key0, uint32_t key1) {
-  double v[RK] = {dE, dM[0], dM[1], dM[2], 0, 0, 0, 0};
-  group_sum8(v, s_red, g, TPR);
-  if (lt == 0) {
-    st[MCG_S_E] += v[0];
-    st[MCG_S_M] += v[1];
-    st[MCG_S_M + 1] += v[2];
-    st[MCG_S_M + 2] += v[3];
-  }
-  dE = 0.0;
-  dM[0] = dM[1] = dM[2] = 0.0;
+__device__ __forceinline__ void tail_rest(const SV& sv, const mcg_graph& G, const mcg_batch& B, const float* s_coup,
+                                          const float* s_cls, double* s_red, double* st, FrameCtx& C, int g, int lt,
+                                          int TPR, int row, float h, int64_t k, const SweepArgs& a, bool meas,
+                                          bool therm_end, bool rec, uint32_t key0, uint32_t key1) {
   if (rec) {
     double r4[4];
     recompute<NC>(sv, G, s_coup, s_cls, h, s_red, g, lt, TPR, r4);
@@ -911,6 +986,70 @@ __device__ __forceinline__ void lds_sweep_tail(const SV& sv, const mcg_graph& G,
   __syncthreads();
 }
 
+template <int NC, typename ST, bool FULL, typename SV>
+__device__ __forceinline__ void lds_sweep_tail(const SV& sv, const mcg_graph& G, const mcg_batch& B, const float* s_coup,
+                                               const float* s_cls, double* s_red, double* st, FrameCtx& C, int g,
+                                               int lt, int TPR, int row, float h, int64_t k, const SweepArgs& a,
+                                               bool meas, bool therm_end, bool rec, double& dE, double* dM,
+                                               uint32_t key0, uint32_t key1) {
+  double v[RK] = {dE, dM[0], dM[1], dM[2], 0, 0, 0, 0};
+  group_sum8(v, s_red, g, TPR);
+  if (lt == 0) {
+    st[MCG_S_E] += v[0];
+    st[MCG_S_M] += v[1];
+    st[MCG_S_M + 1] += v[2];
+    st[MCG_S_M + 2] += v[3];
+  }
+  dE = 0.0;
+  dM[0] = dM[1] = dM[2] = 0.0;
+  tail_rest<NC, ST, FULL>(sv, G, B, s_coup, s_cls, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end, rec, key0,
+                          key1);
+}
+
+// one sweep of a workgroup's replica group: every colour class in turn,
+// own Philox groups (changes counted in fE, fM) and, for a split replica,
+// the other parts' groups this part depends on (changes not counted)
+template <int NC, typename ST, int MODE>
+__device__ __forceinline__ void sweep_colours(const mcg_graph& G, const Spins<ST>& sp, const float* coup,
+                                              const float* clsdat, const int32_t* s_colptr, const uint8_t* s_pid,
+                                              const int32_t* s_pat, int lt, int TPR, float beta, float h, uint64_t kk,
+                                              uint32_t key0, uint32_t key1, const PartInfo& pi, float& fE, float* fM) {
+  constexpr int GD = SitesPerDraw<NC>::G;
+  const int zmax = G.zmax;
+  const bool split = pi.np > 1;
+  for (int c = 0; c < G.ncol; ++c) {
+    const int b = s_colptr[c], cnt = s_colptr[c + 1] - b;
+    const int ngrp = (cnt + GD - 1) / GD;
+    const int q0 = split ? (int)((int64_t)ngrp * pi.part / pi.np) : 0;
+    const int q1 = split ? (int)((int64_t)ngrp * (pi.part + 1) / pi.np) : ngrp;
+    const int nx = split ? pi.hx[2 * c + 1] : 0;
+    const int32_t* xl = split ? pi.halo + pi.hx[2 * c] : nullptr;
+    float xE = 0.0f, xM[3] = {0.0f, 0.0f, 0.0f};
+    if constexpr (MODE == 0) {
+      for (int q = q0 + lt; q < q1; q += TPR)
+        metro_group_pat<NC, ST>(sp, b, cnt, q, c, s_pid, s_pat, zmax, coup, clsdat, beta, h, kk, key0, key1, fE, fM);
+      for (int j = lt; j < nx; j += TPR)
+        metro_group_pat<NC, ST>(sp, b, cnt, xl[j], c, s_pid, s_pat, zmax, coup, clsdat, beta, h, kk, key0, key1, xE,
+                                xM);
+    } else if constexpr (MODE == 1) {
+      for (int q = q0 + lt; q < q1; q += TPR)
+        metro_group<NC>(sp, b, cnt, q, c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, coup, gas(G.cls), clsdat,
+                        beta, h, kk, key0, key1, fE, fM);
+      for (int j = lt; j < nx; j += TPR)
+        metro_group<NC>(sp, b, cnt, xl[j], c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, coup, gas(G.cls),
+                        clsdat, beta, h, kk, key0, key1, xE, xM);
+    } else {
+      for (int q = q0 + lt; q < q1; q += TPR)
+        metro_group<NC>(sp, b, cnt, q, c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, gas(coup), gas(G.cls),
+                        gas(clsdat), beta, h, kk, key0, key1, fE, fM);
+      for (int j = lt; j < nx; j += TPR)
+        metro_group<NC>(sp, b, cnt, xl[j], c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, gas(coup),
+                        gas(G.cls), gas(clsdat), beta, h, kk, key0, key1, xE, xM);
+    }
+    __syncthreads();
+  }
+}
+
 // sweep loop of one LDS block; MODE 0: pattern links (colour-permuted
 // replica), 1: global link table with LDS-staged couplings/classes, 2:
 // global link table with couplings/classes read from HBM (> 256 tensors or
@@ -923,8 +1062,6 @@ __device__ __forceinline__ void lds_sweeps(const mcg_graph& G, const mcg_batch& 
                                            const int32_t* s_pat, double* s_red, double* st, FrameCtx& C, int g,
                                            int lt, int TPR, int row, float beta, float h, uint32_t key0,
                                            uint32_t key1, const PartInfo& pi, double& dE, double* dM) {
-  constexpr int GD = SitesPerDraw<NC>::G;
-  const int zmax = G.zmax;
   const bool split = pi.np > 1;
   dE = 0.0;
   dM[0] = dM[1] = dM[2] = 0.0;
@@ -932,40 +1069,8 @@ __device__ __forceinline__ void lds_sweeps(const mcg_graph& G, const mcg_batch& 
     const int64_t k = a.k_begin + s;
     const uint64_t kk = (uint64_t)(a.philox_base + k);
     float fE = 0.0f, fM[3] = {0.0f, 0.0f, 0.0f};  // this sweep's accepted changes
-    for (int c = 0; c < G.ncol; ++c) {
-      const int b = s_colptr[c], cnt = s_colptr[c + 1] - b;
-      const int ngrp = (cnt + GD - 1) / GD;
-      const int q0 = split ? (int)((int64_t)ngrp * pi.part / pi.np) : 0;
-      const int q1 = split ? (int)((int64_t)ngrp * (pi.part + 1) / pi.np) : ngrp;
-      // own groups, then (split) the other parts' groups this part needs,
-      // whose changes are not counted here
-      const int nx = split ? pi.hx[2 * c + 1] : 0;
-      const int32_t* xl = split ? pi.halo + pi.hx[2 * c] : nullptr;
-      float xE = 0.0f, xM[3] = {0.0f, 0.0f, 0.0f};
-      if constexpr (MODE == 0) {
-        for (int q = q0 + lt; q < q1; q += TPR)
-          metro_group_pat<NC, ST>(sp, b, cnt, q, c, s_pid, s_pat, zmax, coup, clsdat, beta, h, kk, key0, key1, fE,
-                                  fM);
-        for (int j = lt; j < nx; j += TPR)
-          metro_group_pat<NC, ST>(sp, b, cnt, xl[j], c, s_pid, s_pat, zmax, coup, clsdat, beta, h, kk, key0, key1,
-                                  xE, xM);
-      } else if constexpr (MODE == 1) {
-        for (int q = q0 + lt; q < q1; q += TPR)
-          metro_group<NC>(sp, b, cnt, q, c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, coup, gas(G.cls),
-                          clsdat, beta, h, kk, key0, key1, fE, fM);
-        for (int j = lt; j < nx; j += TPR)
-          metro_group<NC>(sp, b, cnt, xl[j], c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, coup, gas(G.cls),
-                          clsdat, beta, h, kk, key0, key1, xE, xM);
-      } else {
-        for (int q = q0 + lt; q < q1; q += TPR)
-          metro_group<NC>(sp, b, cnt, q, c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, gas(coup),
-                          gas(G.cls), gas(clsdat), beta, h, kk, key0, key1, fE, fM);
-        for (int j = lt; j < nx; j += TPR)
-          metro_group<NC>(sp, b, cnt, xl[j], c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, gas(coup),
-                          gas(G.cls), gas(clsdat), beta, h, kk, key0, key1, xE, xM);
-      }
-      __syncthreads();
-    }
+    sweep_colours<NC, ST, MODE>(G, sp, coup, clsdat, s_colptr, s_pid, s_pat, lt, TPR, beta, h, kk, key0, key1, pi, fE,
+                                fM);
     dE += (double)fE;
     dM[0] += (double)fM[0];
     dM[1] += (double)fM[1];
@@ -976,8 +1081,7 @@ __device__ __forceinline__ void lds_sweeps(const mcg_graph& G, const mcg_batch& 
     const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
     if (meas || therm_end || rec || s == a.nsweeps - 1) {
       lds_sweep_tail<NC, ST, FULL>(sp, G, B, coup, clsdat, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end,
-                                   rec, dE,
-                             dM, key0, key1);
+                                   rec, dE, dM, key0, key1);
     }
   }
 }
@@ -986,7 +1090,8 @@ __device__ __forceinline__ void lds_sweeps(const mcg_graph& G, const mcg_batch& 
 // with LDS tables, 2 wide); FULL: spin-derived observables are measured.
 // Both are compile-time so that each instantiation carries only the code it
 // runs (instruction-cache misses dominate the per-launch cost otherwise).
-template <int NC, typename ST, int MODE, bool FULL>
+// PERSIST: a persistent launch (every workgroup resident, see below).
+template <int NC, typename ST, int MODE, bool FULL, bool PERSIST>
 __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __restrict__ items,
                                               const ItemRep* __restrict__ recs, SweepArgs a) {
   const WorkItem w = items[blockIdx.x];
@@ -1022,6 +1127,8 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   double* s_st = (double*)(smem + off);  // R * NSTATE
   double* s_frm = s_st + R * MCG_NSTATE;  // R * FRM
   off += align16((size_t)R * (MCG_NSTATE + FRM) * 8);
+  unsigned char* s_pers = smem + off;  // tempering state of a persistent launch
+  if (PERSIST) off += persist_lds_bytes(B.nrow, B.nrep);
   ST* s_spin = (ST*)(smem + off);
 
   for (int t = threadIdx.x; t < ncl * 9; t += blockDim.x) s_coup[t] = G.coup[t];
@@ -1048,8 +1155,160 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
     pi.hx = a.halo + w.hoff;
     pi.halo = a.halo;
   }
-  double dE, dM[3];
   FrameCtx C{&G, wide ? G.coup : s_coup, wide ? G.clsdat : s_clsl, s_red, s_frm + g * FRM, g, lt, TPR};
+  if constexpr (PERSIST) {
+    // Persistent launch: many sweeps with the replicas in LDS.  After each
+    // sweep every replica's energy is published at the device coherence
+    // point (double-buffered by sweep parity); split parts publish their
+    // partial sums and the sites another part reads (export lists).  A grid
+    // barrier follows whenever tempering is due or parts must exchange.
+    // Then every workgroup computes the swap decisions of every ladder
+    // itself, on its own copy of who/rung (identical in all workgroups:
+    // same energies, same Philox words), so no second barrier is needed:
+    // a workgroup can get at most one sweep ahead, and the parity buffers
+    // keep what the slower ones still read.  Workgroup 0 writes the tables
+    // back at the end.
+    typedef typename RawBits<sizeof(ST)>::T U;
+    constexpr int GD = SitesPerDraw<NC>::G;
+    const int nrow = B.nrow, nrepb = B.nrep;
+    int32_t* s_who = (int32_t*)s_pers;
+    int32_t* s_ptacc = s_who + nrow;
+    int32_t* s_rung = s_ptacc + nrow;
+    float* s_beta = (float*)(s_pers + align16((size_t)(2 * nrow + nrepb) * 4));
+    float* s_h = s_beta + nrow;
+    for (int t = threadIdx.x; t < nrow; t += blockDim.x) {
+      s_who[t] = B.who[t];
+      s_ptacc[t] = 0;
+      s_beta[t] = B.row_beta[t];
+      s_h[t] = B.row_h[t];
+    }
+    for (int t = threadIdx.x; t < nrepb; t += blockDim.x) s_rung[t] = B.rung[t];
+    __syncthreads();
+    U* lraw = (U*)my;
+    U* xb = (U*)a.xbuf;
+    const int DS = halo_desc_ints(G.ncol);
+    int* s_flag = (int*)s_red;
+    int nbar = 0;
+    for (int s = 0; s < a.nsweeps; ++s) {
+      const int64_t k = a.k_begin + s;
+      const uint64_t kk = (uint64_t)(a.philox_base + k);
+      const int par = (int)(k & 1);
+      const int rowg = ir.row0 + s_rung[rep];
+      const float bg = s_beta[rowg], hg = s_h[rowg];
+      float fE = 0.0f, fM[3] = {0.0f, 0.0f, 0.0f};
+      sweep_colours<NC, ST, MODE>(G, sp, C.coup, C.clsdat, s_colptr, s_pid, s_pat, lt, TPR, bg, hg, kk, key0, key1, pi,
+                                  fE, fM);
+      double v[RK] = {(double)fE, (double)fM[0], (double)fM[1], (double)fM[2], 0.0, 0.0, 0.0, 0.0};
+      group_sum8(v, s_red, g, TPR);
+      const bool meas = k >= a.nthermal && ((k - a.nthermal + 1) % a.spf) == 0;
+      const bool therm_end = (k + 1 == a.nthermal);
+      const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
+      const bool pt_now = a.pt_every > 0 && ((k + 1) % a.pt_every) == 0;
+      double* es = a.eslot + ((size_t)par * nrepb + rep) * MAX_PARTS;
+      double* ms = a.mslot + ((size_t)par * nrepb + rep) * MAX_PARTS * 3;
+      if (pi.np == 1) {
+        if (lt == 0) {
+          st[MCG_S_E] += v[0];
+          st[MCG_S_M] += v[1];
+          st[MCG_S_M + 1] += v[2];
+          st[MCG_S_M + 2] += v[3];
+        }
+        tail_rest<NC, ST, false>(sp, G, B, C.coup, C.clsdat, s_red, st, C, g, lt, TPR, rowg, hg, k, a, meas, therm_end,
+                                 rec, key0, key1);
+        if (lt == 0) dev_store(es, st[MCG_S_E]);
+      } else {
+        if (lt == 0) {
+          dev_store(es + pi.part, (pi.part == 0 ? st[MCG_S_E] : 0.0) + v[0]);
+          for (int q = 0; q < 3; ++q) dev_store(ms + pi.part * 3 + q, v[1 + q]);
+        }
+        const int32_t* ex = a.halo + pi.hx[2 * G.ncol];
+        const int nex = pi.hx[2 * G.ncol + 1];
+        U* xo = xb + (((size_t)par * a.nslots + w.slot) * MAX_PARTS + pi.part) * (size_t)a.max_export * NC;
+        for (int i = lt; i < nex; i += TPR)
+          for (int q = 0; q < NC; ++q) dev_store(xo + (size_t)i * NC + q, lraw[(size_t)q * n + ex[i]]);
+      }
+      if (a.has_split || pt_now) {
+        if (!grid_barrier(a.bar, (uint32_t)((int64_t)(a.bar_base + nbar + 1) * gridDim.x), B.err, s_flag)) return;
+        ++nbar;
+      }
+      if (pi.np > 1) {
+        for (int p = 0; p < pi.np; ++p) {
+          if (p == pi.part) continue;
+          const int32_t* hq = pi.hx + (p - pi.part) * DS;
+          const int32_t* ex = a.halo + hq[2 * G.ncol];
+          const int nex = hq[2 * G.ncol + 1];
+          const U* xi = xb + (((size_t)par * a.nslots + w.slot) * MAX_PARTS + p) * (size_t)a.max_export * NC;
+          for (int i = lt; i < nex; i += TPR)
+            for (int q = 0; q < NC; ++q) lraw[(size_t)q * n + ex[i]] = dev_load(xi + (size_t)i * NC + q);
+        }
+        if (pi.part == 0) {
+          if (lt == 0) {
+            double E = st[MCG_S_E] + v[0];
+            double dm[3] = {v[1], v[2], v[3]};
+            for (int p = 1; p < pi.np; ++p) {
+              E += dev_load(es + p);
+              for (int q = 0; q < 3; ++q) dm[q] += dev_load(ms + p * 3 + q);
+            }
+            st[MCG_S_E] = E;
+            for (int q = 0; q < 3; ++q) st[MCG_S_M + q] += dm[q];
+          }
+          tail_rest<NC, ST, false>(sp, G, B, C.coup, C.clsdat, s_red, st, C, g, lt, TPR, rowg, hg, k, a, meas,
+                                   therm_end, false, key0, key1);
+        }
+        __syncthreads();
+      }
+      if (pt_now) {
+        const int64_t swap = a.philox_base + k;
+        const int pp = (int)(swap & 1);
+        const int32_t* pre = a.lad_pre + pp * (a.nladder + 1);
+        const int tot = pre[a.nladder];
+        for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+          int L = 0;
+          while (pre[L + 1] <= i) ++L;
+          const int t = pp + 2 * (i - pre[L]);
+          const int r0 = a.lad_row0[L];
+          const int ra = s_who[r0 + t], rb = s_who[r0 + t + 1];
+          const double* ea = a.eslot + ((size_t)par * nrepb + ra) * MAX_PARTS;
+          const double* eb = a.eslot + ((size_t)par * nrepb + rb) * MAX_PARTS;
+          double Ea = dev_load(ea), Eb = dev_load(eb);
+          for (int q = 1; q < a.rep_np[ra]; ++q) Ea += dev_load(ea + q);
+          for (int q = 1; q < a.rep_np[rb]; ++q) Eb += dev_load(eb + q);
+          const double ba = s_beta[r0 + t], bb = s_beta[r0 + t + 1];
+          const double d = (ba - bb) * (Ea - Eb);
+          bool acc = d >= 0.0;
+          if (!acc) {
+            const u32x4 x =
+                philox4x32_10(u32x4{(uint32_t)t, (uint32_t)swap, (uint32_t)((uint64_t)swap >> 32), TAG_PT},
+                              B.seed_lo, B.seed_hi ^ (0x80000000u | (uint32_t)L));
+            acc = u24(x.x) < exp_neg((float)d);
+          }
+          if (acc) {
+            s_who[r0 + t] = rb;
+            s_who[r0 + t + 1] = ra;
+            s_rung[ra] = t + 1;
+            s_rung[rb] = t;
+            s_ptacc[r0 + t] += 1;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    __syncthreads();
+    if (pi.np == 1)
+      stage<ST, false>(my, gsp, NC * n, lt, TPR);
+    else
+      part_copy<ST, true>(my, gsp, n, NC, s_colptr, G.ncol, GD, pi.part, pi.np, lt, TPR);
+    if (pi.part == 0 && lt < MCG_NSTATE) B.state[(size_t)rep * MCG_NSTATE + lt] = st[lt];
+    if (blockIdx.x == 0) {
+      for (int t = threadIdx.x; t < nrow; t += blockDim.x) {
+        B.who[t] = s_who[t];
+        B.pt_acc[t] += s_ptacc[t];
+      }
+      for (int t = threadIdx.x; t < nrepb; t += blockDim.x) B.rung[t] = s_rung[t];
+    }
+    return;
+  } else {
+  double dE, dM[3];
   if constexpr (MODE == 0)
     lds_sweeps<NC, ST, 0, FULL>(G, B, a, sp, s_coup, s_clsl, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
                                 beta, h, key0, key1, pi, dE, dM);
@@ -1107,9 +1366,9 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
     }
   __syncthreads();
   lds_sweep_tail<NC, ST, FULL>(sp, G, B, C.coup, C.clsdat, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end,
-                               rec, tE,
-                         tM, key0, key1);
+                               rec, tE, tM, key0, key1);
   if (lt < MCG_NSTATE) B.state[(size_t)rep * MCG_NSTATE + lt] = st[lt];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1489,6 +1748,9 @@ struct RunCtx {
   int block = 0;
   int ncu = 0;
   int nslots = 0;  // split replicas (arrival counters)
+  int max_export = 0;  // longest export list of a part
+  bool persist = false;  // persistent launches (barrier + tempering inside)
+  DevBuf bar, eslot, mslot, xbuf, rep_np, lad_pre;
   int mode_begin[4] = {0, 0, 0, 0};  // work items of link mode m: [mode_begin[m], mode_begin[m+1])
   size_t lds_max = 0;
   std::vector<int32_t> hl_row0, hl_n;
@@ -1547,28 +1809,35 @@ int lds_mode(const mcg_graph& G) {
   return wide ? 2 : G.npat > 0 ? 0 : 1;
 }
 
-template <int NC, typename ST, int MODE, bool FULL>
+template <int NC, typename ST, int MODE, bool FULL, bool PERSIST>
 int set_lds_attr(size_t bytes) {
-  hipError_t e = hipFuncSetAttribute((const void*)k_lds<NC, ST, MODE, FULL>,
+  hipError_t e = hipFuncSetAttribute((const void*)k_lds<NC, ST, MODE, FULL, PERSIST>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   (void)e;
   (void)hipGetLastError();
   return 0;
 }
 
-// Extra-group tables of a replica split into np parts (see the comment on
-// split replicas at the LDS kernel): for each part, the other parts' groups
-// of each colour that its own sites depend on, closed under "neighbour of
-// lower colour".  Appends one descriptor per part (2*ncol ints: start,
-// count) and the lists to `halo`; returns the descriptors' offsets, or an
-// empty vector when `limit` is set and a part would recompute more than a
-// quarter of its own groups (then the replica is not worth splitting).
-std::vector<int32_t> halo_tables(const mcg_graph& G, int gd, int np, bool limit, std::vector<int32_t>& halo) {
+// Tables of a replica split into np parts (see the comment on split
+// replicas at the LDS kernel).  For each part p:
+//   * extra groups: the other parts' groups of each colour that p's own
+//     sites depend on, closed under "neighbour of lower colour";
+//   * export list: p's own sites that another part reads during a sweep
+//     (a recomputed site or a neighbour of one), which the persistent kernel
+//     hands over between sweeps.
+// The np descriptors are contiguous, DESC(ncol) ints each: per colour
+// (start, count) of the extra groups, then (start, count) of the export
+// list, all indices into `halo`.  Returns the offset of the first
+// descriptor, or -1 when `limit` is set and a part would recompute more
+// than a quarter of its own groups (then the replica is not worth
+// splitting), or -2 when the link table cannot be read.  *max_export gets
+// the longest export list.
+int halo_tables(const mcg_graph& G, int gd, int np, bool limit, std::vector<int32_t>& halo, int* max_export) {
   const int n = G.n, zmax = G.zmax, ncol = G.ncol;
   std::vector<uint32_t> link((size_t)n * zmax);
   std::vector<int32_t> colptr(ncol + 1);
-  if (zmax > 0 && hipMemcpy(link.data(), G.link, link.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return {};
-  if (hipMemcpy(colptr.data(), G.col_ptr, colptr.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return {};
+  if (zmax > 0 && hipMemcpy(link.data(), G.link, link.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return -2;
+  if (hipMemcpy(colptr.data(), G.col_ptr, colptr.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return -2;
   std::vector<int32_t> col(n), grp(n), own(n);
   for (int c = 0; c < ncol; ++c) {
     const int b = colptr[c], cnt = colptr[c + 1] - b, ngrp = (cnt + gd - 1) / gd;
@@ -1581,21 +1850,31 @@ std::vector<int32_t> halo_tables(const mcg_graph& G, int gd, int np, bool limit,
       }
     }
   }
-  std::vector<int32_t> offs;
-  std::vector<char> need(n);
+  auto nbr = [&](int e, int l) { return (int)(link[(size_t)e * zmax + l] & 0xFFFFFFu); };
+  const int DS = halo_desc_ints(ncol);
+  const size_t base = halo.size();
+  halo.resize(base + (size_t)np * DS);
+  std::vector<char> wanted(n, 0);  // read by a part that does not own it
   for (int part = 0; part < np; ++part) {
+    std::vector<char> need(n);
     for (int e = 0; e < n; ++e) need[e] = own[e] == part;
     for (int c = ncol - 1; c >= 0; --c)
       for (int e = colptr[c]; e < colptr[c + 1]; ++e) {
         if (!need[e]) continue;
         for (int l = 0; l < zmax; ++l) {
-          const int y = (int)(link[(size_t)e * zmax + l] & 0xFFFFFFu);
+          const int y = nbr(e, l);
           if (y != e && col[y] < c) need[y] = 1;
         }
       }
-    const size_t d = halo.size();
-    offs.push_back((int32_t)d);
-    halo.resize(d + 2 * (size_t)ncol);
+    for (int e = 0; e < n; ++e) {
+      if (!need[e]) continue;
+      if (own[e] != part) wanted[e] = 1;
+      for (int l = 0; l < zmax; ++l) {
+        const int y = nbr(e, l);
+        if (own[y] != part) wanted[y] = 1;
+      }
+    }
+    const size_t d = base + (size_t)part * DS;
     int64_t own_groups = 0, extra = 0;
     for (int c = 0; c < ncol; ++c) {
       const int b = colptr[c], cnt = colptr[c + 1] - b, ngrp = (cnt + gd - 1) / gd;
@@ -1611,9 +1890,20 @@ std::vector<int32_t> halo_tables(const mcg_graph& G, int gd, int np, bool limit,
       halo[d + 2 * c + 1] = (int32_t)(halo.size() - start);
       extra += (int64_t)(halo.size() - start);
     }
-    if (limit && extra * 4 > own_groups) return {};
+    if (limit && extra * 4 > own_groups) return -1;
   }
-  return offs;
+  int mx = 0;
+  for (int part = 0; part < np; ++part) {
+    const size_t d = base + (size_t)part * DS;
+    const size_t start = halo.size();
+    for (int e = 0; e < n; ++e)
+      if (wanted[e] && own[e] == part) halo.push_back(e);
+    halo[d + 2 * ncol] = (int32_t)start;
+    halo[d + 2 * ncol + 1] = (int32_t)(halo.size() - start);
+    mx = std::max(mx, (int)(halo.size() - start));
+  }
+  *max_export = std::max(*max_export, mx);
+  return (int)base;
 }
 
 // Plan the LDS work items.
@@ -1734,15 +2024,16 @@ int plan_lds(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool force_stream
       break;
     }
     if (splits.size() > 65535) return fail("too many split replicas");
-    // extra-group tables, one set per split graph
-    std::vector<std::vector<int32_t>> hoffs(B->ngraph);
+    // split tables, one set per split graph
+    std::vector<int> hbase(B->ngraph, -1);
     bool redo = false;
     X.hhalo.clear();
+    X.max_export = 0;
     for (int gi = 0; gi < B->ngraph; ++gi) {
       if (gnp[gi] < 2) continue;
-      hoffs[gi] = halo_tables(B->graphs_host[gi], gd, gnp[gi], P->parts == 0, X.hhalo);
-      if (hoffs[gi].empty()) {
-        if (P->parts >= 2) return fail("graph %d: reading the link table for the split failed", gi);
+      hbase[gi] = halo_tables(B->graphs_host[gi], gd, gnp[gi], P->parts == 0, X.hhalo, &X.max_export);
+      if (hbase[gi] == -2) return fail("graph %d: reading the link table for the split failed", gi);
+      if (hbase[gi] < 0) {
         nosplit[gi] = 1;
         redo = true;
       }
@@ -1763,7 +2054,9 @@ int plan_lds(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool force_stream
         const uint16_t slot = (uint16_t)X.nslots++;
         const int gi = splits[q].gi;
         for (int p = 0; p < np; ++p) {
-          run[(q - j) + (size_t)8 * p] = WorkItem{gi, 1, 0, (uint8_t)p, (uint8_t)np, slot, hoffs[gi][p]};
+          run[(q - j) + (size_t)8 * p] =
+              WorkItem{gi, 1, 0, (uint8_t)p, (uint8_t)np, slot,
+                       hbase[gi] + p * halo_desc_ints(B->graphs_host[gi].ncol)};
           run_rep[(q - j) + (size_t)8 * p] = splits[q].rep;
         }
       }
@@ -1813,13 +2106,15 @@ size_t lds_bytes_for(const mcg_batch* B, const RunCtx& X, int i0, int i1) {
   return mx;
 }
 
-template <int NC, typename ST, int MODE, bool FULL>
+template <int NC, typename ST, int MODE, bool FULL, bool PERSIST>
 int launch_lds_range(const mcg_batch* B, RunCtx& X, const SweepArgs& a, hipStream_t s, int i0, int i1, hipEvent_t e0,
                      hipEvent_t e1) {
-  const size_t bytes = lds_bytes_for<NC, ST>(B, X, i0, i1);
-  set_lds_attr<NC, ST, MODE, FULL>(bytes);
-  hipExtLaunchKernelGGL((k_lds<NC, ST, MODE, FULL>), dim3((unsigned)(i1 - i0)), dim3(X.block), (uint32_t)bytes, s,
-                        e0, e1, 0, *B, (const WorkItem*)X.items.p + i0, (const ItemRep*)X.item_reps.p, a);
+  size_t bytes = lds_bytes_for<NC, ST>(B, X, i0, i1);
+  if (PERSIST) bytes += persist_lds_bytes(B->nrow, B->nrep);
+  set_lds_attr<NC, ST, MODE, FULL, PERSIST>(bytes);
+  hipExtLaunchKernelGGL((k_lds<NC, ST, MODE, FULL, PERSIST>), dim3((unsigned)(i1 - i0)), dim3(X.block),
+                        (uint32_t)bytes, s, e0, e1, 0, *B, (const WorkItem*)X.items.p + i0,
+                        (const ItemRep*)X.item_reps.p, a);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1834,18 +2129,56 @@ int launch_lds(const mcg_batch* B, RunCtx& X, const SweepArgs& a, hipStream_t s,
     hipEvent_t e0, e1;
     if (T.pair(&e0, &e1)) return -1;
     int r;
-    if (m == 0)
-      r = full ? launch_lds_range<NC, ST, 0, true>(B, X, a, s, i0, i1, e0, e1)
-               : launch_lds_range<NC, ST, 0, false>(B, X, a, s, i0, i1, e0, e1);
+    if (X.persist)  // single mode, no spin-derived observables (checked when planned)
+      r = m == 0   ? launch_lds_range<NC, ST, 0, false, true>(B, X, a, s, i0, i1, e0, e1)
+          : m == 1 ? launch_lds_range<NC, ST, 1, false, true>(B, X, a, s, i0, i1, e0, e1)
+                   : launch_lds_range<NC, ST, 2, false, true>(B, X, a, s, i0, i1, e0, e1);
+    else if (m == 0)
+      r = full ? launch_lds_range<NC, ST, 0, true, false>(B, X, a, s, i0, i1, e0, e1)
+               : launch_lds_range<NC, ST, 0, false, false>(B, X, a, s, i0, i1, e0, e1);
     else if (m == 1)
-      r = full ? launch_lds_range<NC, ST, 1, true>(B, X, a, s, i0, i1, e0, e1)
-               : launch_lds_range<NC, ST, 1, false>(B, X, a, s, i0, i1, e0, e1);
+      r = full ? launch_lds_range<NC, ST, 1, true, false>(B, X, a, s, i0, i1, e0, e1)
+               : launch_lds_range<NC, ST, 1, false, false>(B, X, a, s, i0, i1, e0, e1);
     else
-      r = full ? launch_lds_range<NC, ST, 2, true>(B, X, a, s, i0, i1, e0, e1)
-               : launch_lds_range<NC, ST, 2, false>(B, X, a, s, i0, i1, e0, e1);
+      r = full ? launch_lds_range<NC, ST, 2, true, false>(B, X, a, s, i0, i1, e0, e1)
+               : launch_lds_range<NC, ST, 2, false, false>(B, X, a, s, i0, i1, e0, e1);
     if (r) return -1;
     T.launches += 1;
   }
+  return 0;
+}
+
+// Can this run use a persistent launch?  All replicas on the LDS path, one
+// link mode, no spin-derived observables, tempering or split replicas to
+// synchronise (otherwise a plain multi-sweep launch already is persistent),
+// no periodic recompute of split replicas, the tempering state fits in LDS,
+// and every workgroup resident at once.
+template <int NC, typename ST>
+int check_persist(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool* ok) {
+  *ok = false;
+  if (X.hitems.empty() || !X.hstream.empty()) return 0;
+  int modes = 0, m = -1;
+  for (int q = 0; q < 3; ++q)
+    if (X.mode_begin[q + 1] > X.mode_begin[q]) ++modes, m = q;
+  if (modes != 1) return 0;
+  if (B->meas & MEAS_SPIN_OBS) return 0;
+  const bool pt = P->pt_every > 0 && !X.hl_row0.empty();
+  if (!pt && X.nslots == 0) return 0;
+  if (X.nslots > 0 && B->recompute_every > 0) return 0;
+  const size_t bytes = lds_bytes_for<NC, ST>(B, X, 0, (int)X.hitems.size()) + persist_lds_bytes(B->nrow, B->nrep);
+  if (bytes > X.lds_max) return 0;
+  const void* f = m == 0 ? (const void*)k_lds<NC, ST, 0, false, true>
+                  : m == 1 ? (const void*)k_lds<NC, ST, 1, false, true>
+                           : (const void*)k_lds<NC, ST, 2, false, true>;
+  if (m == 0) set_lds_attr<NC, ST, 0, false, true>(bytes);
+  else if (m == 1) set_lds_attr<NC, ST, 1, false, true>(bytes);
+  else set_lds_attr<NC, ST, 2, false, true>(bytes);
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, X.block, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  *ok = (int64_t)per_cu * X.ncu >= (int64_t)X.hitems.size();
   return 0;
 }
 
@@ -2027,12 +2360,58 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
   double lds_updates_per_sweep = 0;
   for (const WorkItem& w : X.hitems)
     if (w.part == 0) lds_updates_per_sweep += (double)w.nrep * B->graphs_host[w.graph].n;
-  // split replicas finish every launch: one sweep per launch
-  const int64_t chunk = X.nslots > 0 ? 1 : P->pt_every > 0 ? P->pt_every : total;
+  // persistent launches when possible (P->no_persist turns them off)
+  X.persist = false;
+  if (!P->no_persist && !force_stream && check_persist<NC, ST>(B, P, X, &X.persist)) return -1;
+  if (X.persist) {
+    std::vector<uint32_t> z(SYNC_STRIDE, 0u);
+    if (X.bar.upload(z.data(), z.size() * 4, s)) return -1;
+    std::vector<double> ze((size_t)2 * B->nrep * MAX_PARTS * 4, 0.0);
+    if (X.eslot.upload(ze.data(), (size_t)2 * B->nrep * MAX_PARTS * 8, s)) return -1;
+    if (X.mslot.upload(ze.data(), (size_t)2 * B->nrep * MAX_PARTS * 3 * 8, s)) return -1;
+    const size_t xb = (size_t)2 * std::max(X.nslots, 1) * MAX_PARTS * std::max(X.max_export, 1) * NC * sizeof(ST);
+    std::vector<unsigned char> zx(xb, 0);
+    if (X.xbuf.upload(zx.data(), xb, s)) return -1;
+    std::vector<uint8_t> np(B->nrep, 1);
+    for (const WorkItem& w : X.hitems)
+      if (w.nparts > 1) np[X.hitem_reps[w.off]] = w.nparts;
+    if (X.rep_np.upload(np.data(), np.size(), s)) return -1;
+    const int nl = (int)X.hl_row0.size();
+    std::vector<int32_t> pre(2 * (nl + 1), 0);
+    for (int pp = 0; pp < 2; ++pp)
+      for (int L = 0; L < nl; ++L) pre[pp * (nl + 1) + L + 1] = pre[pp * (nl + 1) + L] + std::max(0, (X.hl_n[L] - pp) / 2);
+    if (X.lad_pre.upload(pre.data(), pre.size() * 4, s)) return -1;
+  }
+  const bool pt_on = P->pt_every > 0 && !X.hl_row0.empty();
+  // a persistent launch runs up to 4096 sweeps; otherwise split replicas
+  // finish every launch (one sweep per launch) and tempering needs a launch
+  // boundary every pt_every sweeps
+  const int64_t chunk = X.persist ? 4096 : X.nslots > 0 ? 1 : P->pt_every > 0 ? P->pt_every : total;
+  int64_t bar_base = 0;
   for (int64_t k = 0; k < total;) {
     const int64_t ns = std::min<int64_t>(chunk, total - k);
     SweepArgs a{k, P->sweep0, (int32_t)ns, (int32_t)nth, spf, P->frame0, (uint32_t*)X.sync.p, (double*)X.xpart.p,
                 (const int32_t*)X.halo.p};
+    if (X.persist) {
+      a.bar = (uint32_t*)X.bar.p;
+      a.bar_base = (int32_t)bar_base;
+      a.pt_every = pt_on ? P->pt_every : 0;
+      a.has_split = X.nslots > 0;
+      a.nladder = (int32_t)X.hl_row0.size();
+      a.lad_row0 = (const int32_t*)X.lad_row0.p;
+      a.lad_n = (const int32_t*)X.lad_n.p;
+      a.lad_pre = (const int32_t*)X.lad_pre.p;
+      a.rep_np = (const uint8_t*)X.rep_np.p;
+      a.eslot = (double*)X.eslot.p;
+      a.mslot = (double*)X.mslot.p;
+      a.xbuf = X.xbuf.p;
+      a.max_export = std::max(X.max_export, 1);
+      a.nslots = std::max(X.nslots, 1);
+      for (int64_t q = k; q < k + ns; ++q)
+        if (X.nslots > 0 || (pt_on && ((q + 1) % P->pt_every) == 0)) ++bar_base;
+      if (bar_base * (int64_t)X.hitems.size() >= ((int64_t)1 << 31))
+        return fail("too many barriers in one call (%lld); split the call", (long long)bar_base);
+    }
     if (!X.hitems.empty()) {
       if (launch_lds<NC, ST>(B, X, a, s, T)) return -1;
       T.sweeps += (double)ns;
@@ -2040,7 +2419,7 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
     }
     if (stream_sweeps<NC, ST>(B, P, X, k, ns, nth, spf, plain, s, T)) return -1;
     k += ns;
-    if (P->pt_every > 0 && (k % P->pt_every) == 0 && !X.hl_row0.empty()) {
+    if (!X.persist && pt_on && (k % P->pt_every) == 0) {
       hipLaunchKernelGGL(k_pt, dim3((unsigned)X.hl_row0.size()), dim3(64), 0, s, *B,
                          (const int32_t*)X.lad_row0.p, (const int32_t*)X.lad_n.p, (int)X.hl_row0.size(),
                          P->sweep0 + k - 1);

@@ -370,13 +370,16 @@ class Batch:
 
     def run(self, nthermal: int, nsweep: int, sweeps_per_frame: int = 1, algorithm: int = L.METROPOLIS,
             pt_every: int = 0, kernel: int = L.KERNEL_AUTO, block: int = 0, reps_per_block: int = 0,
-            clusters_per_sweep: int = 0, sweep0: int = 0, frame0: int = 0, parts: int = 0) -> None:
+            clusters_per_sweep: int = 0, sweep0: int = 0, frame0: int = 0, parts: int = 0,
+            no_persist: bool = False) -> None:
         """parts: workgroups per LDS replica (0 = balance automatically, 1 =
-        never split, 2..4 = split every LDS replica)."""
+        never split, 2..4 = split every LDS replica).  no_persist: never run
+        the LDS kernel as a persistent launch with tempering inside."""
         p = L.Plan()
         p.algorithm, p.nthermal, p.nsweep, p.sweeps_per_frame = algorithm, nthermal, nsweep, sweeps_per_frame
         p.pt_every, p.kernel, p.block, p.reps_per_block = pt_every, kernel, block, reps_per_block
         p.clusters_per_sweep, p.sweep0, p.frame0, p.parts = clusters_per_sweep, sweep0, frame0, parts
+        p.no_persist = int(bool(no_persist))
         L.check(L.lib().mcg_run(C.byref(self.b), C.byref(p), self._stream()), "mcg_run")
 
     def sweeps(self, sweep0: int, nsweeps: int, algorithm: int = L.METROPOLIS, kernel: int = L.KERNEL_AUTO,

@@ -146,8 +146,10 @@ def test_wolff_bitexact(gpu, case):
         assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
-def test_parallel_tempering_bitexact(gpu):
-    """Ising ladder, swap every sweep: permutation, rungs and spins match."""
+@pytest.mark.parametrize("no_persist", [False, True])
+def test_parallel_tempering_bitexact(gpu, no_persist):
+    """Ising ladder, swap every sweep: permutation, rungs and spins match
+    (persistent launch with tempering inside, and one launch per sweep)."""
     nc, Lt = 1, lat.square(8, ising=True)
     nT = 8
     betas = list(np.linspace(0.25, 0.65, nT).astype(np.float32))
@@ -159,7 +161,7 @@ def test_parallel_tempering_bitexact(gpu):
     b = Batch(1, [g], reps, betas, [0.0] * nT, init, seed=SEED)
     b.init()
     nsw = 10
-    b.run(0, nsw, 1, pt_every=1, kernel=L.KERNEL_LDS)
+    b.run(0, nsw, 1, pt_every=1, kernel=L.KERNEL_LDS, no_persist=no_persist)
     got = _gpu_spins(b, nT)
     # CPU: per sweep, every replica at its current rung's beta, then swap
     ref = _cpu_spins(1, init, "f32")
@@ -291,3 +293,27 @@ def test_auto_split_large_replicas(gpu):
     mcref.cb_sweeps(nc, 0, nbr, J, S, D, g.colour, betas, [0.0, 0.0], np.arange(2, dtype=np.uint32), SEED, 0, 5, ref)
     assert np.array_equal(res[0].view(np.uint32), ref.view(np.uint32))
     assert np.array_equal(res[1].view(np.uint32), ref.view(np.uint32))
+
+
+def _run_ladders(parts, no_persist, pt_every, nth=10, nsw=40, meas=0):
+    graphs, reps, betas, spins = _ladder_batch()
+    b = Batch(2, graphs, reps, betas, [0.02] * len(betas), spins, meas=meas, nbins=4, nframes=nsw)
+    b.init()
+    b.run(nth, nsw, 1, pt_every=pt_every, kernel=L.KERNEL_LDS, parts=parts, no_persist=no_persist)
+    return (np.concatenate([b.replica_spins(k).ravel() for k in range(len(reps))]), b.accumulators(), b.bin_data(),
+            b.pt_acc.cpu().numpy(), b.rung.cpu().numpy(), b.who.cpu().numpy(), b.state_data(), b.error_flag())
+
+
+@pytest.mark.parametrize("parts,pt_every", [(1, 1), (1, 3), (2, 1), (2, 0), (3, 2)])
+def test_persistent_matches_per_launch(gpu, parts, pt_every):
+    """A persistent launch (grid barrier, tempering and part exchange inside
+    the kernel) runs the same trajectory as one launch per sweep."""
+    a = _run_ladders(parts, True, pt_every)
+    b = _run_ladders(parts, False, pt_every)
+    assert a[-1] == 0 and b[-1] == 0
+    assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32))
+    for x, y in zip(a[3:6], b[3:6]):
+        assert np.array_equal(x, y)
+    np.testing.assert_allclose(b[1], a[1], rtol=1e-11, atol=1e-11)
+    np.testing.assert_allclose(b[2], a[2], rtol=1e-11, atol=1e-11)
+    np.testing.assert_allclose(b[6][:, :4], a[6][:, :4], rtol=1e-11, atol=1e-9)
