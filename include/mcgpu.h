@@ -67,6 +67,9 @@ typedef struct mcg_graph {
   int32_t n, zmax, ncol, ncoup, ncls;
   int32_t nlat, ncirc, nr, nclu, ngroup, gmax;
   int32_t npat;            /* link patterns (0 = none: LDS kernel reads link[]) */
+  int32_t ucid;            /* >= 0: every site has zmax links, all with coupling ucid; the
+                              Metropolis field is then J . (sum of neighbour spins) */
+  int32_t pad_;
   const uint32_t *link;    /* [n*zmax] */
   const float *coup;       /* [ncoup*9] row-major 3x3 (E = s_i^T J s_j), unscaled */
   const int32_t *cls;      /* [n] site class */

@@ -142,14 +142,22 @@ static constexpr double REF_PI = 3.1415926535;  // heisenbergLib.c:6
 // renormalised to |S|.
 // Random words: w_a = r_z (Heisenberg), w_b = phi (XY, Heisenberg),
 // w_c = acceptance.
-template <int NC, typename SV, typename LK, typename CP>
+// UNI: every link of the graph carries the tensor at coup (a uniform
+// graph); the field is that tensor times the sum of the neighbour spins.
+template <int NC, typename SV, typename LK, typename CP, bool UNI = false>
 __device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, int zmax, CP coup, int ci, CP clsdat,
                                            float beta, float h, uint32_t w_a, uint32_t w_b, uint32_t w_c,
                                            float& dEacc, float* dMacc) {
   const float S = clsdat[ci * 8 + 0];
   if constexpr (NC == 1) {
     float f = 0.0f;
-    for (int l = 0; l < zmax; ++l) f = fmaf(coup[lk.cid(l) * 9], sp.get(lk.nbr(l), 0), f);
+    if constexpr (UNI) {
+      float m = 0.0f;
+      for (int l = 0; l < zmax; ++l) m += sp.get(lk.nbr(l), 0);
+      f = coup[0] * m;
+    } else {
+      for (int l = 0; l < zmax; ++l) f = fmaf(coup[lk.cid(l) * 9], sp.get(lk.nbr(l), 0), f);
+    }
     const float sg = sp.get(i, 0);
     const float s = sg * S;
     const float t = f - h;
@@ -165,18 +173,35 @@ __device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, in
     const float s0 = sp.get(i, 0), s1 = sp.get(i, 1);
     const float s2 = NC == 3 ? sp.get(i, 2) : 0.0f;
     float H0 = 0.0f, H1 = 0.0f, H2 = 0.0f;
-    for (int l = 0; l < zmax; ++l) {
-      const int j = lk.nbr(l);
-      const CP J = coup + lk.cid(l) * 9;
-      const float t0 = sp.get(j, 0), t1 = sp.get(j, 1);
+    if constexpr (UNI) {
+      float S0 = 0.0f, S1 = 0.0f, S2 = 0.0f;
+      for (int l = 0; l < zmax; ++l) {
+        const int j = lk.nbr(l);
+        S0 += sp.get(j, 0);
+        S1 += sp.get(j, 1);
+        if constexpr (NC == 3) S2 += sp.get(j, 2);
+      }
+      H0 = coup[0] * S0; H0 = fmaf(coup[1], S1, H0);
+      H1 = coup[3] * S0; H1 = fmaf(coup[4], S1, H1);
       if constexpr (NC == 3) {
-        const float t2 = sp.get(j, 2);
-        H0 = fmaf(J[0], t0, H0); H0 = fmaf(J[1], t1, H0); H0 = fmaf(J[2], t2, H0);
-        H1 = fmaf(J[3], t0, H1); H1 = fmaf(J[4], t1, H1); H1 = fmaf(J[5], t2, H1);
-        H2 = fmaf(J[6], t0, H2); H2 = fmaf(J[7], t1, H2); H2 = fmaf(J[8], t2, H2);
-      } else {
-        H0 = fmaf(J[0], t0, H0); H0 = fmaf(J[1], t1, H0);
-        H1 = fmaf(J[3], t0, H1); H1 = fmaf(J[4], t1, H1);
+        H0 = fmaf(coup[2], S2, H0);
+        H1 = fmaf(coup[5], S2, H1);
+        H2 = coup[6] * S0; H2 = fmaf(coup[7], S1, H2); H2 = fmaf(coup[8], S2, H2);
+      }
+    } else {
+      for (int l = 0; l < zmax; ++l) {
+        const int j = lk.nbr(l);
+        const CP J = coup + lk.cid(l) * 9;
+        const float t0 = sp.get(j, 0), t1 = sp.get(j, 1);
+        if constexpr (NC == 3) {
+          const float t2 = sp.get(j, 2);
+          H0 = fmaf(J[0], t0, H0); H0 = fmaf(J[1], t1, H0); H0 = fmaf(J[2], t2, H0);
+          H1 = fmaf(J[3], t0, H1); H1 = fmaf(J[4], t1, H1); H1 = fmaf(J[5], t2, H1);
+          H2 = fmaf(J[6], t0, H2); H2 = fmaf(J[7], t1, H2); H2 = fmaf(J[8], t2, H2);
+        } else {
+          H0 = fmaf(J[0], t0, H0); H0 = fmaf(J[1], t1, H0);
+          H1 = fmaf(J[3], t0, H1); H1 = fmaf(J[4], t1, H1);
+        }
       }
     }
     float rx, ry, rz = 0.0f;
@@ -239,7 +264,7 @@ struct SitesPerDraw {
   static constexpr int G = NC == 1 ? 4 : NC == 2 ? 2 : 1;
 };
 
-template <int NC, typename SV, typename CP>
+template <int NC, bool UNI = false, typename SV, typename CP>
 __device__ __forceinline__ void metro_group(const SV& sp, int b, int cnt, int q, int colour,
                                             const GAS uint32_t* __restrict__ link,
                                             const GAS int32_t* __restrict__ lcid, int zmax, CP coup,
@@ -259,14 +284,15 @@ __device__ __forceinline__ void metro_group(const SV& sp, int b, int cnt, int q,
       const uint32_t wa = NC == 3 ? w[0] : 0u;
       const uint32_t wb = NC == 3 ? w[1] : NC == 2 ? w[2 * g] : 0u;
       const uint32_t wc = NC == 3 ? w[2] : NC == 2 ? w[2 * g + 1] : w[g];
-      metro_core<NC>(sp, i, lk, zmax, coup, (int)cls[i], clsdat, beta, h, wa, wb, wc, dEacc, dMacc);
+      metro_core<NC, SV, GlobalLinks, CP, UNI>(sp, i, lk, zmax, coup, (int)cls[i], clsdat, beta, h, wa, wb, wc, dEacc,
+                                               dMacc);
     }
   }
 }
 
 // same, on the colour-permuted LDS replica: colour c occupies positions
 // b .. b+cnt-1, the neighbours come from the pattern row of each position
-template <int NC, typename ST>
+template <int NC, typename ST, bool UNI = false>
 __device__ __forceinline__ void metro_group_pat(const Spins<ST>& sp, int b, int cnt, int q, int colour,
                                                 const uint8_t* pat, const int32_t* pattab, int zmax,
                                                 const float* coup, const float* clsdat, float beta, float h,
@@ -286,8 +312,8 @@ __device__ __forceinline__ void metro_group_pat(const Spins<ST>& sp, int b, int 
       const uint32_t wa = NC == 3 ? w[0] : 0u;
       const uint32_t wb = NC == 3 ? w[1] : NC == 2 ? w[2 * g] : 0u;
       const uint32_t wc = NC == 3 ? w[2] : NC == 2 ? w[2 * g + 1] : w[g];
-      metro_core<NC, Spins<ST>, PatLinks, const float*>(sp, p, lk, zmax, coup, row[zmax], clsdat, beta, h, wa, wb, wc,
-                                                         dEacc, dMacc);
+      metro_core<NC, Spins<ST>, PatLinks, const float*, UNI>(sp, p, lk, zmax, coup, row[zmax], clsdat, beta, h, wa, wb,
+                                                              wc, dEacc, dMacc);
     }
   }
 }
@@ -1025,26 +1051,52 @@ __device__ __forceinline__ void sweep_colours(const mcg_graph& G, const Spins<ST
     const int nx = split ? pi.hx[2 * c + 1] : 0;
     const int32_t* xl = split ? pi.halo + pi.hx[2 * c] : nullptr;
     float xE = 0.0f, xM[3] = {0.0f, 0.0f, 0.0f};
-    if constexpr (MODE == 0) {
+    if constexpr (MODE == 0 || MODE == 3) {
+      constexpr bool UNI = MODE == 3;
+      const float* cp = UNI ? coup + G.ucid * 9 : coup;
       for (int q = q0 + lt; q < q1; q += TPR)
-        metro_group_pat<NC, ST>(sp, b, cnt, q, c, s_pid, s_pat, zmax, coup, clsdat, beta, h, kk, key0, key1, fE, fM);
+        metro_group_pat<NC, ST, UNI>(sp, b, cnt, q, c, s_pid, s_pat, zmax, cp, clsdat, beta, h, kk, key0, key1, fE,
+                                     fM);
       for (int j = lt; j < nx; j += TPR)
-        metro_group_pat<NC, ST>(sp, b, cnt, xl[j], c, s_pid, s_pat, zmax, coup, clsdat, beta, h, kk, key0, key1, xE,
-                                xM);
+        metro_group_pat<NC, ST, UNI>(sp, b, cnt, xl[j], c, s_pid, s_pat, zmax, cp, clsdat, beta, h, kk, key0, key1,
+                                     xE, xM);
     } else if constexpr (MODE == 1) {
-      for (int q = q0 + lt; q < q1; q += TPR)
-        metro_group<NC>(sp, b, cnt, q, c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, coup, gas(G.cls), clsdat,
-                        beta, h, kk, key0, key1, fE, fM);
-      for (int j = lt; j < nx; j += TPR)
-        metro_group<NC>(sp, b, cnt, xl[j], c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, coup, gas(G.cls),
-                        clsdat, beta, h, kk, key0, key1, xE, xM);
+      const GAS int32_t* lcid = G.lcid ? gas(G.lcid) : nullptr;
+      if (G.ucid >= 0) {
+        const float* cp = coup + G.ucid * 9;
+        for (int q = q0 + lt; q < q1; q += TPR)
+          metro_group<NC, true>(sp, b, cnt, q, c, gas(G.link), lcid, zmax, cp, gas(G.cls), clsdat, beta, h, kk, key0,
+                                key1, fE, fM);
+        for (int j = lt; j < nx; j += TPR)
+          metro_group<NC, true>(sp, b, cnt, xl[j], c, gas(G.link), lcid, zmax, cp, gas(G.cls), clsdat, beta, h, kk,
+                                key0, key1, xE, xM);
+      } else {
+        for (int q = q0 + lt; q < q1; q += TPR)
+          metro_group<NC>(sp, b, cnt, q, c, gas(G.link), lcid, zmax, coup, gas(G.cls), clsdat, beta, h, kk, key0, key1,
+                          fE, fM);
+        for (int j = lt; j < nx; j += TPR)
+          metro_group<NC>(sp, b, cnt, xl[j], c, gas(G.link), lcid, zmax, coup, gas(G.cls), clsdat, beta, h, kk, key0,
+                          key1, xE, xM);
+      }
     } else {
-      for (int q = q0 + lt; q < q1; q += TPR)
-        metro_group<NC>(sp, b, cnt, q, c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, gas(coup), gas(G.cls),
-                        gas(clsdat), beta, h, kk, key0, key1, fE, fM);
-      for (int j = lt; j < nx; j += TPR)
-        metro_group<NC>(sp, b, cnt, xl[j], c, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, zmax, gas(coup),
-                        gas(G.cls), gas(clsdat), beta, h, kk, key0, key1, xE, xM);
+      const GAS int32_t* lcid = G.lcid ? gas(G.lcid) : nullptr;
+      const bool uni = G.ucid >= 0;
+      const GAS float* cp = gas(coup) + (uni ? G.ucid * 9 : 0);
+      if (uni) {
+        for (int q = q0 + lt; q < q1; q += TPR)
+          metro_group<NC, true>(sp, b, cnt, q, c, gas(G.link), lcid, zmax, cp, gas(G.cls), gas(clsdat), beta, h, kk,
+                                key0, key1, fE, fM);
+        for (int j = lt; j < nx; j += TPR)
+          metro_group<NC, true>(sp, b, cnt, xl[j], c, gas(G.link), lcid, zmax, cp, gas(G.cls), gas(clsdat), beta, h,
+                                kk, key0, key1, xE, xM);
+      } else {
+        for (int q = q0 + lt; q < q1; q += TPR)
+          metro_group<NC>(sp, b, cnt, q, c, gas(G.link), lcid, zmax, cp, gas(G.cls), gas(clsdat), beta, h, kk, key0,
+                          key1, fE, fM);
+        for (int j = lt; j < nx; j += TPR)
+          metro_group<NC>(sp, b, cnt, xl[j], c, gas(G.link), lcid, zmax, cp, gas(G.cls), gas(clsdat), beta, h, kk,
+                          key0, key1, xE, xM);
+      }
     }
     __syncthreads();
   }
@@ -1105,7 +1157,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   const int row = ir.row0 + B.rung[rep];
   const float beta = B.row_beta[row], h = B.row_h[row];
   const int nwaves = blockDim.x / 64;
-  const bool perm = MODE == 0;
+  const bool perm = MODE == 0 || MODE == 3;
   const int n = G.n, zmax = G.zmax;
   const bool wide = MODE == 2;
   const int ncl = wide ? 0 : G.ncoup;  // couplings staged in LDS
@@ -1309,9 +1361,9 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
     return;
   } else {
   double dE, dM[3];
-  if constexpr (MODE == 0)
-    lds_sweeps<NC, ST, 0, FULL>(G, B, a, sp, s_coup, s_clsl, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
-                                beta, h, key0, key1, pi, dE, dM);
+  if constexpr (MODE == 0 || MODE == 3)
+    lds_sweeps<NC, ST, MODE, FULL>(G, B, a, sp, s_coup, s_clsl, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
+                                   beta, h, key0, key1, pi, dE, dM);
   else if constexpr (MODE == 1)
     lds_sweeps<NC, ST, 1, FULL>(G, B, a, sp, s_coup, s_clsl, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
                                 beta, h, key0, key1, pi, dE, dM);
@@ -1388,9 +1440,14 @@ __global__ __launch_bounds__(256) void k_pass(mcg_batch B, const int32_t* __rest
     if (q < ngrp) {
       const int row = rp.row0 + B.rung[rep];
       const GSpins<ST> sp{gas((ST*)B.spins + rp.spin_off), G.n};
-      metro_group<NC>(sp, b, cnt, q, colour, gas(G.link), G.lcid ? gas(G.lcid) : nullptr, G.zmax, gas(G.coup),
-                      gas(G.cls), gas(G.clsdat), B.row_beta[row], B.row_h[row], (uint64_t)sweep,
-                      B.seed_lo, B.seed_hi ^ rp.serial, dE, dM);
+      const GAS int32_t* lcid = G.lcid ? gas(G.lcid) : nullptr;
+      if (G.ucid >= 0)
+        metro_group<NC, true>(sp, b, cnt, q, colour, gas(G.link), lcid, G.zmax, gas(G.coup) + G.ucid * 9, gas(G.cls),
+                              gas(G.clsdat), B.row_beta[row], B.row_h[row], (uint64_t)sweep, B.seed_lo,
+                              B.seed_hi ^ rp.serial, dE, dM);
+      else
+        metro_group<NC>(sp, b, cnt, q, colour, gas(G.link), lcid, G.zmax, gas(G.coup), gas(G.cls), gas(G.clsdat),
+                        B.row_beta[row], B.row_h[row], (uint64_t)sweep, B.seed_lo, B.seed_hi ^ rp.serial, dE, dM);
     }
   }
   // block reduction, one FP64 atomic per value per block
@@ -1751,7 +1808,7 @@ struct RunCtx {
   int max_export = 0;  // longest export list of a part
   bool persist = false;  // persistent launches (barrier + tempering inside)
   DevBuf bar, eslot, mslot, xbuf, rep_np, lad_pre;
-  int mode_begin[4] = {0, 0, 0, 0};  // work items of link mode m: [mode_begin[m], mode_begin[m+1])
+  int mode_begin[5] = {0, 0, 0, 0, 0};  // work items of link mode m: [mode_begin[m], mode_begin[m+1])
   size_t lds_max = 0;
   std::vector<int32_t> hl_row0, hl_n;
 };
@@ -1806,7 +1863,7 @@ int device_lds_max(size_t* out) {
 // link mode of a graph on the LDS path (the MODE of k_lds)
 int lds_mode(const mcg_graph& G) {
   const bool wide = G.lcid != nullptr || G.ncls > 256;
-  return wide ? 2 : G.npat > 0 ? 0 : 1;
+  return wide ? 2 : G.npat > 0 ? (G.ucid >= 0 ? 3 : 0) : 1;
 }
 
 template <int NC, typename ST, int MODE, bool FULL, bool PERSIST>
@@ -2080,9 +2137,9 @@ int plan_lds(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool force_stream
     std::stable_sort(out.begin(), out.end(), [&](const WorkItem& x, const WorkItem& y) {
       return lds_mode(B->graphs_host[x.graph]) < lds_mode(B->graphs_host[y.graph]);
     });
-    for (int m = 0; m <= 3; ++m) X.mode_begin[m] = 0;
+    for (int m = 0; m <= 4; ++m) X.mode_begin[m] = 0;
     for (const WorkItem& w : out) X.mode_begin[lds_mode(B->graphs_host[w.graph]) + 1]++;
-    for (int m = 1; m <= 3; ++m) X.mode_begin[m] += X.mode_begin[m - 1];
+    for (int m = 1; m <= 4; ++m) X.mode_begin[m] += X.mode_begin[m - 1];
     X.hitems = out;
     X.hitem_reps = oreps;
     return 0;
@@ -2123,7 +2180,7 @@ int launch_lds_range(const mcg_batch* B, RunCtx& X, const SweepArgs& a, hipStrea
 template <int NC, typename ST>
 int launch_lds(const mcg_batch* B, RunCtx& X, const SweepArgs& a, hipStream_t s, LaunchTimer& T) {
   const bool full = (B->meas & MEAS_SPIN_OBS) != 0;
-  for (int m = 0; m < 3; ++m) {
+  for (int m = 0; m < 4; ++m) {
     const int i0 = X.mode_begin[m], i1 = X.mode_begin[m + 1];
     if (i1 <= i0) continue;
     hipEvent_t e0, e1;
@@ -2132,16 +2189,20 @@ int launch_lds(const mcg_batch* B, RunCtx& X, const SweepArgs& a, hipStream_t s,
     if (X.persist)  // single mode, no spin-derived observables (checked when planned)
       r = m == 0   ? launch_lds_range<NC, ST, 0, false, true>(B, X, a, s, i0, i1, e0, e1)
           : m == 1 ? launch_lds_range<NC, ST, 1, false, true>(B, X, a, s, i0, i1, e0, e1)
-                   : launch_lds_range<NC, ST, 2, false, true>(B, X, a, s, i0, i1, e0, e1);
+          : m == 2 ? launch_lds_range<NC, ST, 2, false, true>(B, X, a, s, i0, i1, e0, e1)
+                   : launch_lds_range<NC, ST, 3, false, true>(B, X, a, s, i0, i1, e0, e1);
     else if (m == 0)
       r = full ? launch_lds_range<NC, ST, 0, true, false>(B, X, a, s, i0, i1, e0, e1)
                : launch_lds_range<NC, ST, 0, false, false>(B, X, a, s, i0, i1, e0, e1);
     else if (m == 1)
       r = full ? launch_lds_range<NC, ST, 1, true, false>(B, X, a, s, i0, i1, e0, e1)
                : launch_lds_range<NC, ST, 1, false, false>(B, X, a, s, i0, i1, e0, e1);
-    else
+    else if (m == 2)
       r = full ? launch_lds_range<NC, ST, 2, true, false>(B, X, a, s, i0, i1, e0, e1)
                : launch_lds_range<NC, ST, 2, false, false>(B, X, a, s, i0, i1, e0, e1);
+    else
+      r = full ? launch_lds_range<NC, ST, 3, true, false>(B, X, a, s, i0, i1, e0, e1)
+               : launch_lds_range<NC, ST, 3, false, false>(B, X, a, s, i0, i1, e0, e1);
     if (r) return -1;
     T.launches += 1;
   }
@@ -2158,7 +2219,7 @@ int check_persist(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool* ok) {
   *ok = false;
   if (X.hitems.empty() || !X.hstream.empty()) return 0;
   int modes = 0, m = -1;
-  for (int q = 0; q < 3; ++q)
+  for (int q = 0; q < 4; ++q)
     if (X.mode_begin[q + 1] > X.mode_begin[q]) ++modes, m = q;
   if (modes != 1) return 0;
   if (B->meas & MEAS_SPIN_OBS) return 0;
@@ -2167,12 +2228,14 @@ int check_persist(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool* ok) {
   if (X.nslots > 0 && B->recompute_every > 0) return 0;
   const size_t bytes = lds_bytes_for<NC, ST>(B, X, 0, (int)X.hitems.size()) + persist_lds_bytes(B->nrow, B->nrep);
   if (bytes > X.lds_max) return 0;
-  const void* f = m == 0 ? (const void*)k_lds<NC, ST, 0, false, true>
+  const void* f = m == 0   ? (const void*)k_lds<NC, ST, 0, false, true>
                   : m == 1 ? (const void*)k_lds<NC, ST, 1, false, true>
-                           : (const void*)k_lds<NC, ST, 2, false, true>;
+                  : m == 2 ? (const void*)k_lds<NC, ST, 2, false, true>
+                           : (const void*)k_lds<NC, ST, 3, false, true>;
   if (m == 0) set_lds_attr<NC, ST, 0, false, true>(bytes);
   else if (m == 1) set_lds_attr<NC, ST, 1, false, true>(bytes);
-  else set_lds_attr<NC, ST, 2, false, true>(bytes);
+  else if (m == 2) set_lds_attr<NC, ST, 2, false, true>(bytes);
+  else set_lds_attr<NC, ST, 3, false, true>(bytes);
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, X.block, bytes) != hipSuccess) {
     (void)hipGetLastError();

@@ -60,6 +60,7 @@ class GraphTables:
     pattab: Optional[np.ndarray] = None
     npat: int = 0
     lcid: Optional[np.ndarray] = None
+    ucid: int = -1
     dev: Optional[dict] = None
     struct: Optional[L.Graph] = None
 
@@ -118,6 +119,9 @@ def compile_graph(nc: int, nbr: np.ndarray, J: np.ndarray, S: np.ndarray, D: np.
     remap[perm] = np.arange(len(perm))
     cid = remap[inv[1:]].reshape(n, zmax)
     wide = len(uniq) > 256
+    # uniform: every site has zmax links and all carry one coupling tensor
+    # (equal FP32 rows); the kernels then sum the neighbour spins first
+    ucid = int(cid[0, 0]) if (zmax > 0 and valid.all() and (cid == cid[0, 0]).all()) else -1
     if wide:
         # per-link coupling indices; link words carry the neighbour only
         lcid = cid.astype(np.int32).reshape(-1)
@@ -218,7 +222,7 @@ def compile_graph(nc: int, nbr: np.ndarray, J: np.ndarray, S: np.ndarray, D: np.
         npat, pattab, pat = 0, np.zeros(1, np.int32), np.zeros(n, np.uint8)
     return GraphTables(n, zmax, link.reshape(-1), uniq.astype(np.float32).reshape(-1), cls_pos, cu.reshape(-1),
                        col_ptr, order, pairs, circ, rorb, rclu, rlk.reshape(-1), garr, gmax, rrank_pos, colour,
-                       pos=pos, pat=pat, pattab=pattab, npat=npat, lcid=lcid)
+                       pos=pos, pat=pat, pattab=pattab, npat=npat, lcid=lcid, ucid=ucid)
 
 
 def _dev(a: np.ndarray, device) -> torch.Tensor:
@@ -247,6 +251,7 @@ def upload_graph(g: GraphTables, device) -> None:
     s.ngroup = 0 if (g.groups < 0).all() else g.groups.shape[0]
     s.gmax = g.gmax
     s.npat = g.npat
+    s.ucid = g.ucid
     for k, t in d.items():
         setattr(s, k, t.data_ptr())
     g.struct = s

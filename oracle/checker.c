@@ -110,6 +110,7 @@ typedef struct {
   const int *nbr;
   float *cj;    /* [n*zmax*9] row-major coupling per link (Ising: [0] folded, [1] raw) */
   float *S, *D; /* [n], [n*3] */
+  int uni;      /* every site has zmax links, all with the tensor cj[0..8] */
 } cb_graph;
 
 static void cb_build(cb_graph *g, int nc, int store, long n, int zmax, const int *nbr, const double *J,
@@ -145,6 +146,14 @@ static void cb_build(cb_graph *g, int nc, int store, long n, int zmax, const int
       }
     }
   }
+  /* uniform graph: the Metropolis field is J . (sum of neighbour spins) */
+  g->uni = zmax > 0;
+  for (long i = 0; i < n && g->uni; i++)
+    for (int l = 0; l < zmax && g->uni; l++) {
+      if (nbr[i * zmax + l] < 0) g->uni = 0;
+      for (int k = 0; k < 9; k++)
+        if (g->cj[(i * zmax + l) * 9 + k] != g->cj[k]) g->uni = 0;
+    }
 }
 
 static void cb_free(cb_graph *g) {
@@ -164,10 +173,16 @@ static void cb_site(const cb_graph *g, float *sp, long i, float beta, float h, u
   const float S = g->S[i];
   if (nc == 1) {
     float f = 0.0f;
-    for (int l = 0; l < g->zmax; l++) {
-      long t = g->nbr[i * g->zmax + l];
-      if (t < 0) t = i;
-      f = fmaf(g->cj[(i * g->zmax + l) * 9], sp[t], f);
+    if (g->uni) {
+      float m = 0.0f;
+      for (int l = 0; l < g->zmax; l++) m += sp[g->nbr[i * g->zmax + l]];
+      f = g->cj[0] * m;
+    } else {
+      for (int l = 0; l < g->zmax; l++) {
+        long t = g->nbr[i * g->zmax + l];
+        if (t < 0) t = i;
+        f = fmaf(g->cj[(i * g->zmax + l) * 9], sp[t], f);
+      }
     }
     float sg = sp[i];
     float s = sg * S;
@@ -183,7 +198,24 @@ static void cb_site(const cb_graph *g, float *sp, long i, float beta, float h, u
   }
   float s0 = sp[i], s1 = sp[n + i], s2 = nc == 3 ? sp[2 * n + i] : 0.0f;
   float H0 = 0, H1 = 0, H2 = 0;
-  for (int l = 0; l < g->zmax; l++) {
+  if (g->uni) {
+    const float *J = g->cj;
+    float S0 = 0, S1 = 0, S2 = 0;
+    for (int l = 0; l < g->zmax; l++) {
+      long t = g->nbr[i * g->zmax + l];
+      S0 += sp[t];
+      S1 += sp[n + t];
+      if (nc == 3) S2 += sp[2 * n + t];
+    }
+    H0 = J[0] * S0; H0 = fmaf(J[1], S1, H0);
+    H1 = J[3] * S0; H1 = fmaf(J[4], S1, H1);
+    if (nc == 3) {
+      H0 = fmaf(J[2], S2, H0);
+      H1 = fmaf(J[5], S2, H1);
+      H2 = J[6] * S0; H2 = fmaf(J[7], S1, H2); H2 = fmaf(J[8], S2, H2);
+    }
+  }
+  for (int l = 0; l < g->zmax && !g->uni; l++) {
     long t = g->nbr[i * g->zmax + l];
     const float *J = g->cj + (i * g->zmax + l) * 9;
     if (t < 0) t = i;

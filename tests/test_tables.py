@@ -120,3 +120,24 @@ def test_wide_coupling_tables():
     for i in range(n):
         for l in range(z):
             assert np.array_equal(coup[cid[i, l]], J[i, l, ROWMAJOR].astype(np.float32))
+
+
+def test_uniform_coupling_flag():
+    """ucid names the one tensor of a graph whose sites all carry zmax links
+    with the same coupling (the Metropolis field is then J times the sum of
+    the neighbour spins); any other graph has ucid -1."""
+    sq = lat.square(8, J=-1.0)
+    g = compile_graph(2, sq.nbr, sq.J, sq.S, sq.D)
+    assert g.ucid >= 0
+    assert np.array_equal(g.coup.reshape(-1, 9)[g.ucid], g.coup.reshape(-1, 9)[int(g.link[0] >> 24)])
+    tri = _triangular_dmi(6)
+    assert compile_graph(3, tri.nbr, tri.J, tri.S, tri.D).ucid == -1
+    # one missing link (an open edge) breaks uniformity
+    nbr = sq.nbr.copy()
+    nbr[0, 0] = -1
+    assert compile_graph(2, nbr, sq.J, sq.S, sq.D).ucid == -1
+    # Ising: the folded coupling J*S_j must agree too
+    S = np.ones(sq.n)
+    S[3] = 2.0
+    assert compile_graph(1, sq.nbr, sq.J[:, :, :1], np.ones(sq.n), np.zeros((sq.n, 3))).ucid >= 0
+    assert compile_graph(1, sq.nbr, sq.J[:, :, :1], S, np.zeros((sq.n, 3))).ucid == -1
