@@ -19,12 +19,13 @@ tempering swap pass + one measured frame.  Printed as one JSON line with:
                sweep (global-memory stencil) and the resulting speedup
 
 python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-plain]
-               [--workload c0|c1|c2|c3|c4] [--store f32|f16]
+               [--workload NAME] [--store f32|f16] [--parts P]
 
---workload selects one of the other reference workloads (C0 Ising 32^2 x 16
-T; C1 Heisenberg honeycomb 64x64x2 x 64 T; C3 triangular DMI 128^2 x 1024
-(T, B); C4 cubic 64^3 x 128 T) and prints the same fields for it (without
-the plain and CPU legs); the default is the headline C2 scan.
+--workload selects one of the other workloads (ising_square32: Ising 32^2 x
+16 T; heis_honeycomb64: Heisenberg honeycomb 64x64x2 x 64 T;
+heis_triangular_dmi128: triangular DMI 128^2 x 1024 (T, B); heis_cubic64:
+cubic 64^3 x 128 T) and prints the same fields for it (without the plain and
+CPU legs); the default is the XY ladder scan above.
 """
 from __future__ import annotations
 
@@ -41,11 +42,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "spin updates/s (replica·sites/s) + % HBM roofline; speedup vs plain at 2σ U4"
 HBM_PEAK = 8.0e12
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lds_xy_c2_hbm_traffic.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lds_xy_scan_hbm_traffic.md")
 
 
 def profiled_traffic():
-    """HBM bytes per launch of the C2 sweep kernel measured with rocprofv3
+    """HBM bytes per launch of the XY-scan sweep kernel measured with rocprofv3
     PMC counters (FETCH_SIZE, WRITE_SIZE, each scaled by a copy-kernel
     calibration), read from the committed profile summary."""
     try:
@@ -103,7 +104,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
-    ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c3", "c4"])
+    ap.add_argument("--workload", default="xy_ladder_scan",
+                    choices=["xy_ladder_scan", "ising_square32", "heis_honeycomb64", "heis_triangular_dmi128",
+                             "heis_cubic64"])
     ap.add_argument("--store", default="f32", choices=["f32", "f16"])
     ap.add_argument("--parts", type=int, default=0,
                     help="workgroups per LDS replica: 0 = balance automatically, 1 = never split")
@@ -114,7 +117,7 @@ def main():
               "so N>1 is the same kernel on 1/N of the batch; multi-GPU runs are not part of this build.",
               file=sys.stderr)
         sys.exit(2)
-    if a.workload != "c2":
+    if a.workload != "xy_ladder_scan":
         other_workload(a)
         return
     # CPU baseline first (forks before the GPU is initialised)
@@ -133,7 +136,7 @@ def main():
         del src, dst
     K, Wm = a.steps, a.warmup
     KT = max(1, K // 4)  # sweeps of the separate per-launch timing pass
-    wl = W.build("c2", nframes=K + Wm + KT, nbins=16)
+    wl = W.build("xy_ladder_scan", nframes=K + Wm + KT, nbins=16)
     b = wl.batch
     b.init()
     b.run(0, Wm, 1, pt_every=wl.pt_every, block=a.block, parts=a.parts)
@@ -179,12 +182,12 @@ def main():
     out = {"metric": METRIC, "value": value, "unit": "spin updates/s", "n_gpus": 1, "steps": K, "warmup": Wm,
            "ms_per_step": ms / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32", "data": "synthetic",
-           "config": {"workload": "configs[2] XY batched Binder scan", "lattices": list(W.C2_SIZES),
+           "config": {"workload": "XY Binder scan, 7 square lattices x 64-rung PT ladders", "lattices": list(W.XY_SCAN_SIZES),
                       "rungs_per_ladder": 64, "replicas": len(b.reps), "spins_per_sweep": ups, "pt_every": 1,
                       "measure_every": 1, "spin_storage": "fp32", "accumulators": "fp64"},
            "roofline": roof, "cpu_baseline": cpu}
     if not a.no_plain:
-        wp = W.build("c2", nframes=a.plain_steps, nbins=1)
+        wp = W.build("xy_ladder_scan", nframes=a.plain_steps, nbins=1)
         wp.batch.init()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -199,7 +202,7 @@ def main():
 
 
 def other_workload(a):
-    """One of C0, C1, C3, C4: K measured sweeps of every replica (no
+    """One of the other workloads: K measured sweeps of every replica (no
     tempering, so one launch runs all sweeps on the LDS path), then a K/4
     sweep pass with per-launch HIP events for the roofline."""
     import torch
@@ -209,7 +212,7 @@ def other_workload(a):
     torch.cuda.set_device(0)
     K, Wm = a.steps, a.warmup
     KT = max(1, K // 4)
-    meas = L.MEAS_PAIRS | (L.MEAS_Q if a.workload == "c3" else 0)
+    meas = L.MEAS_PAIRS | (L.MEAS_Q if a.workload == "heis_triangular_dmi128" else 0)
     wl = W.build(a.workload, store=a.store, nframes=K + Wm + KT, nbins=16, meas=meas)
     b = wl.batch
     b.init()

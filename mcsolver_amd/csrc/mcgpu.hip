@@ -409,25 +409,45 @@ __device__ __forceinline__ double wave_sum_f64(double x) {
   return x;
 }
 
-// sum of RK values over the TPR threads of group g (TPR a multiple of 64);
-// the result is valid in the group's first thread.  All threads call it.
-__device__ __forceinline__ void group_sum8(double* v, double* s_red, int g, int TPR) {
+// lane 63's value, in every lane of the wave
+__device__ __forceinline__ double lane63(double x) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), 63);
+  return __hiloint2double(hi, lo);
+}
+
+// sum of N (<= RK) values over the TPR threads of group g (TPR a multiple of
+// 64); the result is valid in every lane of the group's first wave, so in
+// the group's first thread.  All threads call it; it ends with a workgroup
+// barrier.  Two levels of DPP wave sums: the waves' totals go through LDS
+// and the group's first wave adds them up (at most 16 waves).
+template <int N>
+__device__ __forceinline__ void group_sum(double* v, double* s_red, int g, int TPR) {
 #pragma unroll
-  for (int k = 0; k < RK; ++k) v[k] = wave_sum_f64(v[k]);
+  for (int k = 0; k < N; ++k) v[k] = wave_sum_f64(v[k]);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = TPR >> 6;
+  if (nw == 1) {  // the same for every group of the workgroup
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = lane63(v[k]);
+    __syncthreads();
+    return;
+  }
   if (lane == 63)
-    for (int k = 0; k < RK; ++k) s_red[wave * RK + k] = v[k];
+#pragma unroll
+    for (int k = 0; k < N; ++k) s_red[wave * RK + k] = v[k];
   __syncthreads();
-  if ((int)threadIdx.x == g * TPR) {
-    const int w0 = g * TPR / 64, nw = TPR / 64;
-    for (int k = 0; k < RK; ++k) {
-      double t = 0.0;
-      for (int q = 0; q < nw; ++q) t += s_red[(w0 + q) * RK + k];
-      v[k] = t;
+  const int w0 = g * nw;
+  if (wave == w0) {  // the whole wave, so that the DPP sums see every lane
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      const double x = lane < nw ? s_red[(w0 + lane) * RK + k] : 0.0;
+      v[k] = lane63(wave_sum_f64(x));
     }
   }
   __syncthreads();
 }
+__device__ __forceinline__ void group_sum8(double* v, double* s_red, int g, int TPR) { group_sum<RK>(v, s_red, g, TPR); }
 
 // ---------------------------------------------------------------------------
 // measurement of one frame (all flagged observables) for one replica
@@ -501,9 +521,57 @@ __device__ __forceinline__ float signed_area(const float* a, const float* b, con
 // groups); a kernel compiled without them (FULL = false) is much smaller.
 static constexpr int MEAS_SPIN_OBS = MCG_MEAS_PAIRS | MCG_MEAS_Q | MCG_MEAS_RNORM | MCG_MEAS_GROUPS;
 
+template <int NC>
+__device__ __forceinline__ int frame_groups(const mcg_graph& G, int flags) {
+  return (NC > 1 && (flags & MCG_MEAS_GROUPS)) ? G.ngroup : 0;
+}
+
+// apply a frame staged by measure_frame to the accumulators of its row: one
+// device-scope FP64 atomic per non-zero term, spread over the group's
+// threads.  Fire-and-forget: nothing here waits for the atomics.
+template <int NC>
+__device__ __forceinline__ void flush_frame(const FrameCtx& C, const mcg_batch& B, int row, int flags) {
+  const mcg_graph& G = *C.G;
+  const int lt = C.lt, TPR = C.TPR;
+  const double* F = C.s_frm;
+  const double* Cn = F + FRM_ACC;
+  const int ngq = frame_groups<NC>(G, flags);
+  double* A = B.acc + (size_t)row * MCG_NACC;
+  const int nge = ngq > 0 ? (ngq + 1) * (ngq + 1) : 0;
+  for (int q = lt; q < MCG_NACC + MCG_NBINQ + nge; q += TPR) {
+    if (q < MCG_NACC) {
+      if (q != MCG_A_MPREV && Cn[q] != 0.0) dev_add(A + q, Cn[q]);  // adding 0 changes nothing
+    } else if (q < MCG_NACC + MCG_NBINQ) {
+      const int64_t bin = (int64_t)Cn[MCG_NACC + MCG_NBINQ];
+      if (Cn[q] != 0.0) dev_add(B.bins + ((size_t)row * B.nbins + bin) * MCG_NBINQ + (q - MCG_NACC), Cn[q]);
+    } else {
+      // orbital groups (heisenbergLib.c:805-830): <g_a . g_b> and <(g_a . g_a)^2>
+      const int j = q - MCG_NACC - MCG_NBINQ, ga = j / (ngq + 1), gb = j % (ngq + 1);
+      double* GA = B.gacc + (size_t)row * ((ngq + 2) * (ngq + 1));
+      const double* va = F + 32 + 3 * ga;
+      const double* vb = F + 32 + 3 * gb;
+      const double d = va[0] * vb[0] + va[1] * vb[1] + va[2] * vb[2];
+      dev_add(GA + j, d);
+      if (ga == gb) dev_add(GA + (ngq + 1) * (ngq + 1) + ga, d * d);
+    }
+  }
+}
+
+// the previous frame's M of a row (for <M M'>), read ahead of the frame by
+// the thread that applies it
+__device__ __forceinline__ double frame_mprev(const mcg_batch& B, int row) {
+  return dev_load(B.acc + (size_t)row * MCG_NACC + MCG_A_MPREV);
+}
+
+// One measured frame of the group's replica at `row`: the observables are
+// computed and staged in the group's frame area, <M M'> and M' are brought
+// up to date at once (thread MCG_A_MM; mprev, if have_prev, is M' read ahead
+// by that thread), and the rest is applied to the accumulators now or, with
+// defer, left staged for the caller's flush_frame.
 template <int NC, typename SV, bool FULL = true>
 __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& B, const double* st, int row,
-                              int64_t frame, uint32_t key0, uint32_t key1, float h) {
+                              int64_t frame, uint32_t key0, uint32_t key1, float h, double mprev = 0.0,
+                              bool have_prev = false, bool defer = false) {
   const mcg_graph& G = *C.G;
   const int lt = C.lt, TPR = C.TPR;
   double* F = C.s_frm;  // leader-only use
@@ -541,7 +609,7 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
       v[0] += (double)signed_area(a, b, c, fabsf(C.clsdat[G.cls[ia] * 8]), fabsf(C.clsdat[G.cls[ib] * 8]),
                                   fabsf(C.clsdat[G.cls[ic] * 8]));
     }
-    group_sum8(v, C.s_red, C.g, TPR);
+    group_sum<1>(v, C.s_red, C.g, TPR);
     if (lt == 0) F[8] = v[0];
   }
   // --- block-spin observables
@@ -605,10 +673,10 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
              C.clsdat[ci * 8 + 3] * s[2] * s[2] - h * (NC == 3 ? s[2] : s[0]);
       w[4] += (double)eo;
     }
-    group_sum8(v, C.s_red, C.g, TPR);
+    group_sum<7>(v, C.s_red, C.g, TPR);
     if (lt == 0)
       for (int k = 0; k < 7; ++k) F[16 + k] = v[k];
-    group_sum8(w, C.s_red, C.g, TPR);
+    group_sum<5>(w, C.s_red, C.g, TPR);
     if (lt == 0)
       for (int k = 0; k < 5; ++k) F[24 + k] = w[k];
   }
@@ -625,7 +693,7 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
         v[3 * q] += s[0]; v[3 * q + 1] += s[1]; v[3 * q + 2] += s[2];
       }
     }
-    group_sum8(v, C.s_red, C.g, TPR);
+    group_sum<6>(v, C.s_red, C.g, TPR);
     if (lt == 0)
       for (int k = 0; k < 6; ++k) F[32 + 3 * g0 + k] = v[k];
   }
@@ -635,7 +703,7 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
   //     staged in LDS; the group's threads then apply them to global memory
   //     in parallel (one read-modify-write each instead of a serial chain)
   double* Cn = F + FRM_ACC;  // [0, NACC) accumulators, [NACC, +NBINQ) bin, then bin index
-  const int ngq = (NC > 1 && (flags & MCG_MEAS_GROUPS)) ? G.ngroup : 0;
+  const int ngq = frame_groups<NC>(G, flags);
   if (lt == 0) {
     for (int q = 0; q < MCG_NACC + MCG_NBINQ + 1; ++q) Cn[q] = 0.0;
     const double n = G.n;
@@ -725,33 +793,17 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
   __syncthreads();
   // A row's accumulators are written by whichever replica holds the row, so
   // within one persistent launch by workgroups on different XCDs: the
-  // updates go to the device coherence point (device-scope atomics).
-  double* A = B.acc + (size_t)row * MCG_NACC;
-  const int nge = ngq > 0 ? (ngq + 1) * (ngq + 1) : 0;
-  for (int q = lt; q < MCG_NACC + MCG_NBINQ + nge; q += TPR) {
-    if (q < MCG_NACC) {
-      if (q == MCG_A_MPREV) continue;
-      if (q == MCG_A_MM) {
-        const double prev = dev_load(A + MCG_A_MPREV);
-        dev_add(A + MCG_A_MM, prev * Cn[MCG_A_MM]);
-        dev_store(A + MCG_A_MPREV, Cn[MCG_A_MM]);
-      } else if (Cn[q] != 0.0) {  // adding 0 changes nothing
-        dev_add(A + q, Cn[q]);
-      }
-    } else if (q < MCG_NACC + MCG_NBINQ) {
-      const int64_t bin = (int64_t)Cn[MCG_NACC + MCG_NBINQ];
-      if (Cn[q] != 0.0) dev_add(B.bins + ((size_t)row * B.nbins + bin) * MCG_NBINQ + (q - MCG_NACC), Cn[q]);
-    } else {
-      // orbital groups (heisenbergLib.c:805-830): <g_a . g_b> and <(g_a . g_a)^2>
-      const int j = q - MCG_NACC - MCG_NBINQ, ga = j / (ngq + 1), gb = j % (ngq + 1);
-      double* GA = B.gacc + (size_t)row * ((ngq + 2) * (ngq + 1));
-      const double* va = F + 32 + 3 * ga;
-      const double* vb = F + 32 + 3 * gb;
-      const double d = va[0] * vb[0] + va[1] * vb[1] + va[2] * vb[2];
-      dev_add(GA + j, d);
-      if (ga == gb) dev_add(GA + (ngq + 1) * (ngq + 1) + ga, d * d);
-    }
+  // updates go to the device coherence point (device-scope atomics).  M' is
+  // the one value read back: its store must be complete before the row's
+  // next frame (a workgroup barrier or the launch boundary orders it).
+  if (lt == MCG_A_MM) {
+    double* A = B.acc + (size_t)row * MCG_NACC;
+    const double prev = have_prev ? mprev : dev_load(A + MCG_A_MPREV);
+    const double Mf = Cn[MCG_A_MM];
+    Cn[MCG_A_MM] = prev * Mf;
+    dev_store(A + MCG_A_MPREV, Mf);
   }
+  if (!defer) flush_frame<NC>(C, B, row, flags);
 }
 
 // full FP64 recomputation of E and M of one replica (group-parallel)
@@ -768,7 +820,7 @@ __device__ void recompute(const SV& sp, const mcg_graph& G, const float* coup, c
     v[2] += s[1];
     v[3] += s[2];
   }
-  group_sum8(v, s_red, g, TPR);
+  group_sum<4>(v, s_red, g, TPR);
   if (lt == 0)
     for (int k = 0; k < 4; ++k) out4[k] = v[k];
 }
@@ -992,7 +1044,8 @@ template <int NC, typename ST, bool FULL, typename SV>
 __device__ __forceinline__ void tail_rest(const SV& sv, const mcg_graph& G, const mcg_batch& B, const float* s_coup,
                                           const float* s_cls, double* s_red, double* st, FrameCtx& C, int g, int lt,
                                           int TPR, int row, float h, int64_t k, const SweepArgs& a, bool meas,
-                                          bool therm_end, bool rec, uint32_t key0, uint32_t key1) {
+                                          bool therm_end, bool rec, uint32_t key0, uint32_t key1,
+                                          double mprev = 0.0, bool have_prev = false, bool defer = false) {
   if (rec) {
     double r4[4];
     recompute<NC>(sv, G, s_coup, s_cls, h, s_red, g, lt, TPR, r4);
@@ -1008,7 +1061,8 @@ __device__ __forceinline__ void tail_rest(const SV& sv, const mcg_graph& G, cons
   if (therm_end && lt == 0) st[MCG_S_EREF] = st[MCG_S_E];
   __syncthreads();
   if (meas)
-    measure_frame<NC, SV, FULL>(sv, C, B, st, row, (k - a.nthermal + 1) / a.spf - 1 + a.frame0, key0, key1, h);
+    measure_frame<NC, SV, FULL>(sv, C, B, st, row, (k - a.nthermal + 1) / a.spf - 1 + a.frame0, key0, key1, h, mprev,
+                                have_prev, defer);
   __syncthreads();
 }
 
@@ -1017,9 +1071,10 @@ __device__ __forceinline__ void lds_sweep_tail(const SV& sv, const mcg_graph& G,
                                                const float* s_cls, double* s_red, double* st, FrameCtx& C, int g,
                                                int lt, int TPR, int row, float h, int64_t k, const SweepArgs& a,
                                                bool meas, bool therm_end, bool rec, double& dE, double* dM,
-                                               uint32_t key0, uint32_t key1) {
+                                               uint32_t key0, uint32_t key1, double mprev = 0.0,
+                                               bool have_prev = false) {
   double v[RK] = {dE, dM[0], dM[1], dM[2], 0, 0, 0, 0};
-  group_sum8(v, s_red, g, TPR);
+  group_sum<4>(v, s_red, g, TPR);
   if (lt == 0) {
     st[MCG_S_E] += v[0];
     st[MCG_S_M] += v[1];
@@ -1029,7 +1084,7 @@ __device__ __forceinline__ void lds_sweep_tail(const SV& sv, const mcg_graph& G,
   dE = 0.0;
   dM[0] = dM[1] = dM[2] = 0.0;
   tail_rest<NC, ST, FULL>(sv, G, B, s_coup, s_cls, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end, rec, key0,
-                          key1);
+                          key1, mprev, have_prev);
 }
 
 // one sweep of a workgroup's replica group: every colour class in turn,
@@ -1120,6 +1175,9 @@ __device__ __forceinline__ void lds_sweeps(const mcg_graph& G, const mcg_batch& 
   for (int s = 0; s < a.nsweeps; ++s) {
     const int64_t k = a.k_begin + s;
     const uint64_t kk = (uint64_t)(a.philox_base + k);
+    const bool meas = k >= a.nthermal && ((k - a.nthermal + 1) % a.spf) == 0;
+    double mprev = 0.0;  // read ahead, so that the frame does not wait for it
+    if (meas && !split && lt == MCG_A_MM) mprev = frame_mprev(B, row);
     float fE = 0.0f, fM[3] = {0.0f, 0.0f, 0.0f};  // this sweep's accepted changes
     sweep_colours<NC, ST, MODE>(G, sp, coup, clsdat, s_colptr, s_pid, s_pat, lt, TPR, beta, h, kk, key0, key1, pi, fE,
                                 fM);
@@ -1128,12 +1186,11 @@ __device__ __forceinline__ void lds_sweeps(const mcg_graph& G, const mcg_batch& 
     dM[1] += (double)fM[1];
     dM[2] += (double)fM[2];
     if (split) continue;  // one sweep per launch; the last part to finish runs the tail
-    const bool meas = k >= a.nthermal && ((k - a.nthermal + 1) % a.spf) == 0;
     const bool therm_end = (k + 1 == a.nthermal);
     const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
     if (meas || therm_end || rec || s == a.nsweeps - 1) {
       lds_sweep_tail<NC, ST, FULL>(sp, G, B, coup, clsdat, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end,
-                                   rec, dE, dM, key0, key1);
+                                   rec, dE, dM, key0, key1, mprev, true);
     }
   }
 }
@@ -1247,12 +1304,17 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
       const int par = (int)(k & 1);
       const int rowg = ir.row0 + s_rung[rep];
       const float bg = s_beta[rowg], hg = s_h[rowg];
+      const bool meas = k >= a.nthermal && ((k - a.nthermal + 1) % a.spf) == 0;
+      // M' of the row, read ahead: the row's last frame was taken before the
+      // previous barrier (a split replica's frame is taken after it, so its
+      // part 0 reads M' in the frame itself)
+      double mprev = 0.0;
+      if (meas && pi.np == 1 && lt == MCG_A_MM) mprev = frame_mprev(B, rowg);
       float fE = 0.0f, fM[3] = {0.0f, 0.0f, 0.0f};
       sweep_colours<NC, ST, MODE>(G, sp, C.coup, C.clsdat, s_colptr, s_pid, s_pat, lt, TPR, bg, hg, kk, key0, key1, pi,
                                   fE, fM);
       double v[RK] = {(double)fE, (double)fM[0], (double)fM[1], (double)fM[2], 0.0, 0.0, 0.0, 0.0};
-      group_sum8(v, s_red, g, TPR);
-      const bool meas = k >= a.nthermal && ((k - a.nthermal + 1) % a.spf) == 0;
+      group_sum<4>(v, s_red, g, TPR);
       const bool therm_end = (k + 1 == a.nthermal);
       const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
       const bool pt_now = a.pt_every > 0 && ((k + 1) % a.pt_every) == 0;
@@ -1265,8 +1327,10 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
           st[MCG_S_M + 1] += v[2];
           st[MCG_S_M + 2] += v[3];
         }
+        // the frame's accumulator updates wait until after the barrier
+        // (flush below), so that the barrier does not wait for them
         tail_rest<NC, ST, false>(sp, G, B, C.coup, C.clsdat, s_red, st, C, g, lt, TPR, rowg, hg, k, a, meas, therm_end,
-                                 rec, key0, key1);
+                                 rec, key0, key1, mprev, true, true);
         if (lt == 0) dev_store(es, st[MCG_S_E]);
       } else {
         if (lt == 0) {
@@ -1344,6 +1408,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
         }
         __syncthreads();
       }
+      if (meas && pi.np == 1) flush_frame<NC>(C, B, rowg, B.meas & ~MEAS_SPIN_OBS);
     }
     __syncthreads();
     if (pi.np == 1)
@@ -1392,7 +1457,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   double* xp = a.xpart + (size_t)w.slot * MAX_PARTS * 4;
   part_copy<ST, true>(my, gsp, n, NC, s_colptr, G.ncol, GD, pi.part, pi.np, lt, TPR);
   double v[RK] = {dE, dM[0], dM[1], dM[2], 0.0, 0.0, 0.0, 0.0};
-  group_sum8(v, s_red, g, TPR);
+  group_sum<4>(v, s_red, g, TPR);
   if (lt == 0)
     for (int q = 0; q < 4; ++q) dev_store(xp + pi.part * 4 + q, v[q]);
   stores_done();

@@ -1,18 +1,18 @@
 """The benchmark / parity workloads, built on the engine.
 
-C0  Ising ferromagnet, square L=32 periodic, J=1 (reference input Jxx=-1),
+ising_square32          Ising ferromagnet, square L=32 periodic, J=1 (reference input Jxx=-1),
     h=0, 16 temperatures linear in [1.5, 3.5], all-up start, int8 spins.
-C1  Heisenberg honeycomb (CrI3-like, 2-site basis) 64x64 cells: 3 first- and
+heis_honeycomb64        Heisenberg honeycomb (CrI3-like, 2-site basis) 64x64 cells: 3 first- and
     6 second-neighbour bonds, each a full 3x3 tensor (diagonal -1 +- 0.1,
     off-diagonal U[-0.05, 0.05], seeded), single-ion Dz = -0.1, 64
     temperatures in [0.1, 2.0]; FP32 or FP16 spin storage.
-C2  XY square lattices L in {16,24,32,48,64,96,128} in one launch, each a
+xy_ladder_scan          XY square lattices L in {16,24,32,48,64,96,128} in one launch, each a
     64-rung parallel-tempering ladder over T in [0.7, 1.2] (swap every
     sweep), Jxx = Jyy = -1 (ferro), FP32 spins, FP64 accumulators.
-C3  Heisenberg triangular 128x128, J = -1, Neel DMI D = 0.3 (D = -D0 z x r),
+heis_triangular_dmi128  Heisenberg triangular 128x128, J = -1, Neel DMI D = 0.3 (D = -D0 z x r),
     Dz = -0.05, Bz in [0, 1] x 32 and T in [0.05, 1.0] x 32 (1024 replicas),
     topological charge from 2 triangles per cell.
-C4  Heisenberg cubic 64^3, J = -1, random start, 128 temperatures in
+heis_cubic64            Heisenberg cubic 64^3, J = -1, random start, 128 temperatures in
     [1.2, 1.7]; batched Metropolis, or Wolff on 16 replicas.
 
 Couplings follow the reference's sign convention (negative J is
@@ -30,7 +30,7 @@ from . import lattice as lat
 from .engine import Batch, ReplicaSpec, compile_graph
 
 SEED = 0x5EED2026
-C2_SIZES = (16, 24, 32, 48, 64, 96, 128)
+XY_SCAN_SIZES = (16, 24, 32, 48, 64, 96, 128)
 
 
 @dataclass
@@ -57,7 +57,7 @@ def _graph(nc, Lt, meas_pairs=True):
                          circuits=Lt.circuits if nc == 3 and len(Lt.circuits) else None)
 
 
-def honeycomb_c1(L: int, seed: int = SEED) -> lat.Lattice:
+def honeycomb_full_tensor(L: int, seed: int = SEED) -> lat.Lattice:
     rng = np.random.default_rng(seed)
     bonds = []
     for (s, t, o) in [(0, 1, (0, 0, 0)), (0, 1, (-1, 0, 0)), (0, 1, (0, 1, 0)), (0, 0, (1, 0, 0)), (0, 0, (1, 1, 0)),
@@ -70,7 +70,7 @@ def honeycomb_c1(L: int, seed: int = SEED) -> lat.Lattice:
                              [[1 / 3, 2 / 3, 0], [2 / 3, 1 / 3, 0]], bonds, [1.0, 1.0], [[0, 0, -0.1]] * 2)
 
 
-def triangular_dmi_c3(L: int, D0: float = 0.3, Dz: float = -0.05) -> lat.Lattice:
+def triangular_neel_dmi(L: int, D0: float = 0.3, Dz: float = -0.05) -> lat.Lattice:
     a1, a2 = np.array([1.0, 0, 0]), np.array([0.5, np.sqrt(3) / 2, 0])
     bonds = []
     for o in [(1, 0, 0), (0, 1, 0), (-1, 1, 0)]:
@@ -96,27 +96,27 @@ def _init(nc, n, S, mode, rng):
 
 
 def build(name: str, store: str = "f32", nbins: int = 32, nframes: int = 1, meas: int = 0,
-          sizes=C2_SIZES, pt: bool = True, nrep_c4: int = 128, wolff: bool = False, device="cuda") -> Workload:
+          sizes=XY_SCAN_SIZES, pt: bool = True, nrep_cubic: int = 128, wolff: bool = False, device="cuda") -> Workload:
     rng = np.random.default_rng(SEED)
     reps: List[ReplicaSpec] = []
     graphs, lattices, temps, fields, rgraph, inits = [], [], [], [], [], []
-    nc = {"c0": 1, "c1": 3, "c2": 2, "c3": 3, "c4": 3}[name]
+    nc = {"ising_square32": 1, "heis_honeycomb64": 3, "xy_ladder_scan": 2, "heis_triangular_dmi128": 3, "heis_cubic64": 3}[name]
     pt_every = 0
-    if name == "c0":
+    if name == "ising_square32":
         Lt = lat.square(32, J=-1.0, ising=True)
         graphs.append(_graph(1, Lt)); lattices.append(Lt)
         for k, T in enumerate(np.linspace(1.5, 3.5, 16)):
             reps.append(ReplicaSpec(0, k)); temps.append(T); fields.append(0.0); rgraph.append(0)
             inits.append(_init(1, Lt.n, Lt.S, "x", rng))
         desc = "Ising square L=32, J=1 ferro, h=0, 16 T in [1.5,3.5], int8"
-    elif name == "c1":
-        Lt = honeycomb_c1(64)
+    elif name == "heis_honeycomb64":
+        Lt = honeycomb_full_tensor(64)
         graphs.append(_graph(3, Lt)); lattices.append(Lt)
         for k, T in enumerate(np.linspace(0.1, 2.0, 64)):
             reps.append(ReplicaSpec(0, k)); temps.append(T); fields.append(0.0); rgraph.append(0)
             inits.append(_init(3, Lt.n, Lt.S, "x", rng))
         desc = "Heisenberg honeycomb 64x64x2, 3 NN + 6 NNN full-tensor bonds, Dz=-0.1, 64 T in [0.1,2.0], %s" % store
-    elif name == "c2":
+    elif name == "xy_ladder_scan":
         row = 0
         for gi, Ls in enumerate(sizes):
             Lt = lat.square(Ls, J=-1.0)
@@ -133,8 +133,8 @@ def build(name: str, store: str = "f32", nbins: int = 32, nframes: int = 1, meas
         pt_every = 1 if pt else 0
         desc = "XY square L in %s mixed in one launch, 64-rung PT ladders T in [0.7,1.2], swap every sweep, fp32" % (
             list(sizes),)
-    elif name == "c3":
-        Lt = triangular_dmi_c3(128)
+    elif name == "heis_triangular_dmi128":
+        Lt = triangular_neel_dmi(128)
         graphs.append(_graph(3, Lt)); lattices.append(Lt)
         k = 0
         for B in np.linspace(0.0, 1.0, 32):
@@ -142,13 +142,13 @@ def build(name: str, store: str = "f32", nbins: int = 32, nframes: int = 1, meas
                 reps.append(ReplicaSpec(0, k)); temps.append(T); fields.append(B); rgraph.append(0)
                 inits.append(_init(3, Lt.n, Lt.S, "x", rng)); k += 1
         desc = "Heisenberg triangular 128x128, J=-1, Neel DMI 0.3, Dz=-0.05, 32 Bz x 32 T, %s" % store
-    elif name == "c4":
+    elif name == "heis_cubic64":
         Lt = lat.cubic(64, J=-1.0)
         graphs.append(_graph(3, Lt)); lattices.append(Lt)
-        for k, T in enumerate(np.linspace(1.2, 1.7, nrep_c4)):
+        for k, T in enumerate(np.linspace(1.2, 1.7, nrep_cubic)):
             reps.append(ReplicaSpec(0, k)); temps.append(T); fields.append(0.0); rgraph.append(0)
             inits.append(_init(3, Lt.n, Lt.S, "random", rng))
-        desc = "Heisenberg cubic 64^3, J=-1, random start, %d T in [1.2,1.7]" % nrep_c4
+        desc = "Heisenberg cubic 64^3, J=-1, random start, %d T in [1.2,1.7]" % nrep_cubic
     else:
         raise ValueError(name)
     temps = np.asarray(temps)

@@ -6,7 +6,7 @@ import exact
 from mcsolver_amd import _lib as L
 from mcsolver_amd import lattice as lat
 from mcsolver_amd.engine import Batch, ReplicaSpec, compile_graph
-from mcsolver_amd.workloads import triangular_dmi_c3
+from mcsolver_amd.workloads import triangular_neel_dmi
 
 pytestmark = pytest.mark.gpu
 
@@ -37,7 +37,7 @@ def _q_numpy(Lt, s):
 
 @pytest.mark.parametrize("texture", ["skyrmion", "antiskyrmion_field", "uniform"])
 def test_topological_charge(gpu, texture):
-    Lt = triangular_dmi_c3(48)
+    Lt = triangular_neel_dmi(48)
     if texture == "uniform":
         s = np.zeros((Lt.n, 3)); s[:, 2] = 1
     else:

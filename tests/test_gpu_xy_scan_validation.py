@@ -1,4 +1,4 @@
-"""configs[2]-style batched XY scan vs the FP64 oracle (reference algorithm).
+"""Batched XY ladder scan (square lattices, 64-rung PT ladders) vs the FP64 oracle (reference algorithm).
 
 The batched persistent engine (LDS kernel, PT every sweep) and the plain
 one-launch-per-replica-per-colour path run the same trajectory bit for bit
@@ -36,8 +36,8 @@ def _oracle_point(args):
     return L_, T, u4, Lt.n * (e2 - e * e) / T ** 2
 
 
-def test_c2_batched_matches_oracle(gpu):
-    wl = W.build("c2", sizes=SIZES, nbins=20, nframes=NSW)
+def test_xy_scan_batched_matches_oracle(gpu):
+    wl = W.build("xy_ladder_scan", sizes=SIZES, nbins=20, nframes=NSW)
     b = wl.batch
     b.init()
     b.run(NTH, NSW, 1, pt_every=1)
