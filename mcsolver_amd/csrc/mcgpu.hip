@@ -121,12 +121,28 @@ struct GlobalLinks {
   const GAS int32_t* lc;  // per-link coupling indices (wide graphs) or nullptr
   __device__ __forceinline__ int nbr(int l) const { return (int)(lk[l] & 0xFFFFFFu); }
   __device__ __forceinline__ uint32_t cid(int l) const { return lc ? (uint32_t)lc[l] : lk[l] >> 24; }
+  // neighbours of links l .. l+3
+  __device__ __forceinline__ void nbr4(int l, int* j) const {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) j[u] = nbr(l + u);
+  }
 };
+// LDS pattern rows are padded to a multiple of four words (16 bytes), so
+// that four links come in one 128-bit read
+__host__ __device__ constexpr int pat_stride(int zmax) { return (zmax + 4) & ~3; }
 struct PatLinks {
-  const int32_t* row;
+  const int32_t* row;  // 16-byte aligned
   int p;
   __device__ __forceinline__ int nbr(int l) const { return p + ((int)((uint32_t)row[l] << 8) >> 8); }
   __device__ __forceinline__ uint32_t cid(int l) const { return (uint32_t)row[l] >> 24; }
+  // neighbours of links l .. l+3 (l a multiple of 4)
+  __device__ __forceinline__ void nbr4(int l, int* j) const {
+    const int4 w = *(const int4*)(row + l);
+    j[0] = p + ((int)((uint32_t)w.x << 8) >> 8);
+    j[1] = p + ((int)((uint32_t)w.y << 8) >> 8);
+    j[2] = p + ((int)((uint32_t)w.z << 8) >> 8);
+    j[3] = p + ((int)((uint32_t)w.w << 8) >> 8);
+  }
 };
 
 static constexpr double REF_PI = 3.1415926535;  // heisenbergLib.c:6
@@ -152,8 +168,18 @@ __device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, in
   if constexpr (NC == 1) {
     float f = 0.0f;
     if constexpr (UNI) {
+      // links four at a time: the neighbour indices are read together, then
+      // the spins, so that each site waits for two LDS round trips per four
+      // links rather than two per link (the sum keeps link order)
       float m = 0.0f;
-      for (int l = 0; l < zmax; ++l) m += sp.get(lk.nbr(l), 0);
+      int l = 0;
+      for (; l + 4 <= zmax; l += 4) {
+        int j[4];
+        lk.nbr4(l, j);
+        const float a0 = sp.get(j[0], 0), a1 = sp.get(j[1], 0), a2 = sp.get(j[2], 0), a3 = sp.get(j[3], 0);
+        m += a0; m += a1; m += a2; m += a3;
+      }
+      for (; l < zmax; ++l) m += sp.get(lk.nbr(l), 0);
       f = coup[0] * m;
     } else {
       for (int l = 0; l < zmax; ++l) f = fmaf(coup[lk.cid(l) * 9], sp.get(lk.nbr(l), 0), f);
@@ -174,8 +200,27 @@ __device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, in
     const float s2 = NC == 3 ? sp.get(i, 2) : 0.0f;
     float H0 = 0.0f, H1 = 0.0f, H2 = 0.0f;
     if constexpr (UNI) {
+      // links four at a time, as for Ising
       float S0 = 0.0f, S1 = 0.0f, S2 = 0.0f;
-      for (int l = 0; l < zmax; ++l) {
+      int l = 0;
+      for (; l + 4 <= zmax; l += 4) {
+        int j[4];
+        lk.nbr4(l, j);
+        float t0[4], t1[4], t2[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          t0[u] = sp.get(j[u], 0);
+          t1[u] = sp.get(j[u], 1);
+          t2[u] = NC == 3 ? sp.get(j[u], 2) : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          S0 += t0[u];
+          S1 += t1[u];
+          if constexpr (NC == 3) S2 += t2[u];
+        }
+      }
+      for (; l < zmax; ++l) {
         const int j = lk.nbr(l);
         S0 += sp.get(j, 0);
         S1 += sp.get(j, 1);
@@ -307,7 +352,7 @@ __device__ __forceinline__ void metro_group_pat(const Spins<ST>& sp, int b, int 
     const int lp = q * G + g;
     if (lp < cnt) {
       const int p = b + lp;
-      const int32_t* row = pattab + (int)pat[p] * (zmax + 1);
+      const int32_t* row = pattab + (int)pat[p] * pat_stride(zmax);
       const PatLinks lk{row, p};
       const uint32_t wa = NC == 3 ? w[0] : 0u;
       const uint32_t wb = NC == 3 ? w[1] : NC == 2 ? w[2 * g] : 0u;
@@ -868,7 +913,8 @@ struct SweepArgs {
   int32_t nladder;
   const int32_t* lad_row0;
   const int32_t* lad_n;
-  const int32_t* lad_pre;    // [2][nladder+1] pairs per parity, prefix sums
+  const int32_t* blk_lad;    // per workgroup: [nblocks+1] offsets, then ladder
+                             // indices (bit 31: the workgroup owns the ladder)
   const uint8_t* rep_np;     // parts of each replica (1: whole)
   double* eslot;             // [2][nrep][MAX_PARTS] published energies
   double* mslot;             // [2][nrep][MAX_PARTS][3] published magnetisation changes
@@ -921,7 +967,7 @@ __device__ __forceinline__ void stage(ST* lds, ST* g, int count, int lt, int TPR
 __host__ __device__ inline size_t lds_fixed_bytes(int ncoup, int ncls, int ncol, int npat, int zmax, int n,
                                                   int nwaves, int R) {
   size_t b = align16((size_t)(ncoup * 9 + ncls * 8) * 4) + align16((size_t)(ncol + 1) * 4);
-  if (npat > 0) b += align16((size_t)npat * (zmax + 1) * 4) + align16((size_t)n);
+  if (npat > 0) b += align16((size_t)npat * pat_stride(zmax) * 4) + align16((size_t)n);
   return b + align16((size_t)nwaves * RK * 8) + align16((size_t)R * (MCG_NSTATE + FRM) * 8);
 }
 
@@ -963,24 +1009,43 @@ __device__ __forceinline__ void stores_done() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-// Grid-wide barrier of a persistent launch (every workgroup resident):
-// count in, wait until the counter reaches target (the barrier's index times
-// the workgroup count).  Bounded: on time-out bit 16 of the error flag is
-// raised and false returned, and the caller leaves the kernel.
-__device__ __forceinline__ bool grid_barrier(uint32_t* bar, uint32_t target, int32_t* err, int* s_flag) {
+// Grid-wide barrier of a persistent launch (every workgroup resident), in
+// two levels, because arrivals on one counter serialise at its memory
+// channel (256 workgroups on one counter: ~3.9 us per barrier on MI355X;
+// 16 counters of 16 plus a top counter: ~1.6 us).  Workgroup b counts in
+// on group counter b % BAR_GROUPS; the group's last arrival counts in on the
+// top counter; the last of those writes the barrier's epoch (its index,
+// from 1) to every group's release word, which the group's workgroups poll.
+// All words sit in their own 128-byte lines.  Bounded: on time-out bit 16
+// of the error flag is raised and false returned, and the caller leaves the
+// kernel.
+static constexpr int BAR_GROUPS = 16;
+static constexpr int BAR_LINE = 32;  // uint32 per line
+static constexpr int BAR_WORDS = (1 + 2 * BAR_GROUPS) * BAR_LINE;
+__device__ __forceinline__ bool grid_barrier(uint32_t* bar, uint32_t epoch, int32_t* err, int* s_flag) {
   stores_done();
   __syncthreads();
   if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int nb = (int)gridDim.x;
+    const int grp = (int)blockIdx.x % BAR_GROUPS;
+    const int ngrp = nb < BAR_GROUPS ? nb : BAR_GROUPS;
+    const uint32_t gsz = (uint32_t)((nb - grp + BAR_GROUPS - 1) / BAR_GROUPS);
+    uint32_t* gc = bar + (1 + grp) * BAR_LINE;
+    uint32_t* rel = bar + (1 + BAR_GROUPS + grp) * BAR_LINE;
+    const uint32_t old = __hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old + 1 == epoch * gsz) {
+      const uint32_t o2 = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (o2 + 1 == epoch * (uint32_t)ngrp)
+        for (int q = 0; q < ngrp; ++q) dev_store(bar + (1 + BAR_GROUPS + q) * BAR_LINE, epoch);
+    }
     int ok = 1;
     uint32_t it = 0;
-    while ((int32_t)(dev_load(bar) - target) < 0) {
-      if (++it > (1u << 20)) {
+    while ((int32_t)(dev_load(rel) - epoch) < 0) {
+      if (++it > (1u << 22)) {
         atomicOr(err, 16);
         ok = 0;
         break;
       }
-      __builtin_amdgcn_s_sleep(2);
     }
     *s_flag = ok;
   }
@@ -1228,7 +1293,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   int32_t* s_colptr = (int32_t*)(smem + off);
   off += align16((size_t)(G.ncol + 1) * 4);
   int32_t* s_pat = (int32_t*)(smem + off);
-  if (perm) off += align16((size_t)G.npat * (zmax + 1) * 4);
+  if (perm) off += align16((size_t)G.npat * pat_stride(zmax) * 4);
   uint8_t* s_pid = (uint8_t*)(smem + off);
   if (perm) off += align16((size_t)n);
   double* s_red = (double*)(smem + off);
@@ -1244,7 +1309,8 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   for (int t = threadIdx.x; t < ncs * 8; t += blockDim.x) s_clsl[t] = G.clsdat[t];
   for (int t = threadIdx.x; t <= G.ncol; t += blockDim.x) s_colptr[t] = G.col_ptr[t];
   if (perm) {
-    for (int t = threadIdx.x; t < G.npat * (zmax + 1); t += blockDim.x) s_pat[t] = G.pattab[t];
+    for (int t = threadIdx.x; t < G.npat * (zmax + 1); t += blockDim.x)
+      s_pat[(t / (zmax + 1)) * pat_stride(zmax) + t % (zmax + 1)] = G.pattab[t];
     for (int t = threadIdx.x; t < n; t += blockDim.x) s_pid[t] = G.pat[t];
   }
 
@@ -1344,7 +1410,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
           for (int q = 0; q < NC; ++q) dev_store(xo + (size_t)i * NC + q, lraw[(size_t)q * n + ex[i]]);
       }
       if (a.has_split || pt_now) {
-        if (!grid_barrier(a.bar, (uint32_t)((int64_t)(a.bar_base + nbar + 1) * gridDim.x), B.err, s_flag)) return;
+        if (!grid_barrier(a.bar, (uint32_t)(a.bar_base + nbar + 1), B.err, s_flag)) return;
         ++nbar;
       }
       if (pi.np > 1) {
@@ -1374,21 +1440,33 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
         __syncthreads();
       }
       if (pt_now) {
+        // the swaps of the ladders this workgroup's replicas belong to (the
+        // other ladders' rows are never read here)
         const int64_t swap = a.philox_base + k;
         const int pp = (int)(swap & 1);
-        const int32_t* pre = a.lad_pre + pp * (a.nladder + 1);
-        const int tot = pre[a.nladder];
-        for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-          int L = 0;
-          while (pre[L + 1] <= i) ++L;
-          const int t = pp + 2 * (i - pre[L]);
-          const int r0 = a.lad_row0[L];
+        const int li0 = a.blk_lad[blockIdx.x], li1 = a.blk_lad[blockIdx.x + 1];
+        for (int li = li0; li < li1; ++li) {
+        const int L = a.blk_lad[li] & 0x7FFFFFFF;
+        const int r0 = a.lad_row0[L];
+        const int npairs = max(0, (a.lad_n[L] - pp) / 2);
+        for (int i = threadIdx.x; i < npairs; i += blockDim.x) {
+          const int t = pp + 2 * i;
           const int ra = s_who[r0 + t], rb = s_who[r0 + t + 1];
           const double* ea = a.eslot + ((size_t)par * nrepb + ra) * MAX_PARTS;
           const double* eb = a.eslot + ((size_t)par * nrepb + rb) * MAX_PARTS;
-          double Ea = dev_load(ea), Eb = dev_load(eb);
-          for (int q = 1; q < a.rep_np[ra]; ++q) Ea += dev_load(ea + q);
-          for (int q = 1; q < a.rep_np[rb]; ++q) Eb += dev_load(eb + q);
+          // every part slot at once (slots past a replica's parts hold 0)
+          double xa[MAX_PARTS], xb[MAX_PARTS];
+#pragma unroll
+          for (int q = 0; q < MAX_PARTS; ++q) {
+            xa[q] = dev_load(ea + q);
+            xb[q] = dev_load(eb + q);
+          }
+          double Ea = xa[0], Eb = xb[0];
+#pragma unroll
+          for (int q = 1; q < MAX_PARTS; ++q) {
+            Ea += xa[q];
+            Eb += xb[q];
+          }
           const double ba = s_beta[r0 + t], bb = s_beta[r0 + t + 1];
           const double d = (ba - bb) * (Ea - Eb);
           bool acc = d >= 0.0;
@@ -1406,6 +1484,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
             s_ptacc[r0 + t] += 1;
           }
         }
+        }
         __syncthreads();
       }
       if (meas && pi.np == 1) flush_frame<NC>(C, B, rowg, B.meas & ~MEAS_SPIN_OBS);
@@ -1416,12 +1495,19 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
     else
       part_copy<ST, true>(my, gsp, n, NC, s_colptr, G.ncol, GD, pi.part, pi.np, lt, TPR);
     if (pi.part == 0 && lt < MCG_NSTATE) B.state[(size_t)rep * MCG_NSTATE + lt] = st[lt];
-    if (blockIdx.x == 0) {
-      for (int t = threadIdx.x; t < nrow; t += blockDim.x) {
-        B.who[t] = s_who[t];
-        B.pt_acc[t] += s_ptacc[t];
+    // each ladder's state goes back from its owner (flag bit 31 of the
+    // workgroup's ladder entry): the first workgroup holding one of its
+    // replicas
+    for (int li = a.blk_lad[blockIdx.x]; li < a.blk_lad[blockIdx.x + 1]; ++li) {
+      if (!(a.blk_lad[li] & 0x80000000)) continue;
+      const int L = a.blk_lad[li] & 0x7FFFFFFF;
+      const int r0 = a.lad_row0[L];
+      for (int t = threadIdx.x; t < a.lad_n[L]; t += blockDim.x) {
+        const int r = s_who[r0 + t];
+        B.who[r0 + t] = r;
+        B.pt_acc[r0 + t] += s_ptacc[r0 + t];
+        B.rung[r] = s_rung[r];
       }
-      for (int t = threadIdx.x; t < nrepb; t += blockDim.x) B.rung[t] = s_rung[t];
     }
     return;
   } else {
@@ -1872,7 +1958,7 @@ struct RunCtx {
   int nslots = 0;  // split replicas (arrival counters)
   int max_export = 0;  // longest export list of a part
   bool persist = false;  // persistent launches (barrier + tempering inside)
-  DevBuf bar, eslot, mslot, xbuf, rep_np, lad_pre;
+  DevBuf bar, eslot, mslot, xbuf, rep_np, blk_lad;
   int mode_begin[5] = {0, 0, 0, 0, 0};  // work items of link mode m: [mode_begin[m], mode_begin[m+1])
   size_t lds_max = 0;
   std::vector<int32_t> hl_row0, hl_n;
@@ -2492,7 +2578,7 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
   X.persist = false;
   if (!P->no_persist && !force_stream && check_persist<NC, ST>(B, P, X, &X.persist)) return -1;
   if (X.persist) {
-    std::vector<uint32_t> z(SYNC_STRIDE, 0u);
+    std::vector<uint32_t> z(BAR_WORDS, 0u);
     if (X.bar.upload(z.data(), z.size() * 4, s)) return -1;
     std::vector<double> ze((size_t)2 * B->nrep * MAX_PARTS * 4, 0.0);
     if (X.eslot.upload(ze.data(), (size_t)2 * B->nrep * MAX_PARTS * 8, s)) return -1;
@@ -2504,11 +2590,30 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
     for (const WorkItem& w : X.hitems)
       if (w.nparts > 1) np[X.hitem_reps[w.off]] = w.nparts;
     if (X.rep_np.upload(np.data(), np.size(), s)) return -1;
-    const int nl = (int)X.hl_row0.size();
-    std::vector<int32_t> pre(2 * (nl + 1), 0);
-    for (int pp = 0; pp < 2; ++pp)
-      for (int L = 0; L < nl; ++L) pre[pp * (nl + 1) + L + 1] = pre[pp * (nl + 1) + L] + std::max(0, (X.hl_n[L] - pp) / 2);
-    if (X.lad_pre.upload(pre.data(), pre.size() * 4, s)) return -1;
+    // ladders of each workgroup's replicas; the first workgroup of a ladder
+    // owns it
+    std::vector<int32_t> lad_of_row(B->nrow, -1);
+    for (size_t L = 0; L < X.hl_row0.size(); ++L) lad_of_row[X.hl_row0[L]] = (int32_t)L;
+    const size_t nb = X.hitems.size();
+    std::vector<int32_t> bl(nb + 1, 0);
+    std::vector<char> owned(X.hl_row0.size(), 0);
+    for (size_t i = 0; i < nb; ++i) {
+      const WorkItem& w = X.hitems[i];
+      std::vector<int32_t> ls;
+      for (int q = 0; q < w.nrep; ++q) {
+        const mcg_replica& R = B->reps_host[X.hitem_reps[w.off + q]];
+        const int L = R.ladder >= 0 && R.nrung >= 2 ? lad_of_row[R.row0] : -1;
+        if (L >= 0 && std::find(ls.begin(), ls.end(), L) == ls.end()) ls.push_back(L);
+      }
+      std::sort(ls.begin(), ls.end());
+      for (int32_t L : ls) {
+        bl.push_back(owned[L] ? L : (int32_t)(L | 0x80000000u));
+        owned[L] = 1;
+      }
+      bl[i + 1] = (int32_t)(bl.size() - (nb + 1));
+    }
+    for (size_t i = 0; i <= nb; ++i) bl[i] += (int32_t)(nb + 1);
+    if (X.blk_lad.upload(bl.data(), bl.size() * 4, s)) return -1;
   }
   const bool pt_on = P->pt_every > 0 && !X.hl_row0.empty();
   // a persistent launch runs up to 4096 sweeps; otherwise split replicas
@@ -2528,7 +2633,7 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
       a.nladder = (int32_t)X.hl_row0.size();
       a.lad_row0 = (const int32_t*)X.lad_row0.p;
       a.lad_n = (const int32_t*)X.lad_n.p;
-      a.lad_pre = (const int32_t*)X.lad_pre.p;
+      a.blk_lad = (const int32_t*)X.blk_lad.p;
       a.rep_np = (const uint8_t*)X.rep_np.p;
       a.eslot = (double*)X.eslot.p;
       a.mslot = (double*)X.mslot.p;
@@ -2537,7 +2642,7 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
       a.nslots = std::max(X.nslots, 1);
       for (int64_t q = k; q < k + ns; ++q)
         if (X.nslots > 0 || (pt_on && ((q + 1) % P->pt_every) == 0)) ++bar_base;
-      if (bar_base * (int64_t)X.hitems.size() >= ((int64_t)1 << 31))
+      if (bar_base * (int64_t)std::max<size_t>(X.hitems.size(), BAR_GROUPS) >= ((int64_t)1 << 31))
         return fail("too many barriers in one call (%lld); split the call", (long long)bar_base);
     }
     if (!X.hitems.empty()) {

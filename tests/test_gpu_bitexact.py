@@ -295,8 +295,27 @@ def test_auto_split_large_replicas(gpu):
     assert np.array_equal(res[1].view(np.uint32), ref.view(np.uint32))
 
 
-def _run_ladders(parts, no_persist, pt_every, nth=10, nsw=40, meas=0):
-    graphs, reps, betas, spins = _ladder_batch()
+def _ladder_batch_shared():
+    """Three ladders, two of them on one graph (XY L=16: rungs 0-3 and 4-7,
+    so that one workgroup holds replicas of both), one on L=24."""
+    graphs, reps, betas, spins = [], [], [], []
+    rng = np.random.default_rng(12)
+    for gi, L_ in enumerate((16, 24)):
+        Lt = lat.square(L_, J=-1.0, D=(0.05, 0.0, 0.0))
+        graphs.append(compile_graph(2, Lt.nbr, Lt.J, Lt.S, Lt.D, pairs=Lt.corr_pairs))
+        for r in range(8):
+            if gi == 0:
+                reps.append(ReplicaSpec(0, 4 * (r // 4), nrung=4, ladder=r // 4, rung=r % 4))
+            else:
+                reps.append(ReplicaSpec(1, 8, nrung=8, ladder=2, rung=r))
+            betas.append(1 / (0.7 + 0.07 * r))
+            v = rng.normal(size=(Lt.n, 2))
+            spins.append(v / np.linalg.norm(v, axis=1, keepdims=True))
+    return graphs, reps, betas, spins
+
+
+def _run_ladders(parts, no_persist, pt_every, nth=10, nsw=40, meas=0, build=None):
+    graphs, reps, betas, spins = (build or _ladder_batch)()
     b = Batch(2, graphs, reps, betas, [0.02] * len(betas), spins, meas=meas, nbins=4, nframes=nsw)
     b.init()
     b.run(nth, nsw, 1, pt_every=pt_every, kernel=L.KERNEL_LDS, parts=parts, no_persist=no_persist)
@@ -317,3 +336,18 @@ def test_persistent_matches_per_launch(gpu, parts, pt_every):
     np.testing.assert_allclose(b[1], a[1], rtol=1e-11, atol=1e-11)
     np.testing.assert_allclose(b[2], a[2], rtol=1e-11, atol=1e-11)
     np.testing.assert_allclose(b[6][:, :4], a[6][:, :4], rtol=1e-11, atol=1e-9)
+
+
+@pytest.mark.parametrize("parts", [1, 2])
+def test_persistent_shared_graph_ladders(gpu, parts):
+    """Ladders that share a graph (and workgroups): the persistent launch
+    takes each workgroup's ladders from its replicas and writes every
+    ladder back once."""
+    a = _run_ladders(parts, True, 1, build=_ladder_batch_shared)
+    b = _run_ladders(parts, False, 1, build=_ladder_batch_shared)
+    assert a[-1] == 0 and b[-1] == 0
+    assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32))
+    for x, y in zip(a[3:6], b[3:6]):
+        assert np.array_equal(x, y)
+    assert a[3].sum() > 0
+    np.testing.assert_allclose(b[1], a[1], rtol=1e-11, atol=1e-11)
