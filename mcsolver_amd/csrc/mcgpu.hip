@@ -608,6 +608,102 @@ __device__ __forceinline__ double frame_mprev(const mcg_batch& B, int row) {
   return dev_load(B.acc + (size_t)row * MCG_NACC + MCG_A_MPREV);
 }
 
+// One frame's contribution to every FP64 accumulator of a row
+// (isingLib.c:388-431, heisenbergLib.c:677-830 / xyLib.c:605-757), staged in
+// Cn for flush_frame: [0, NACC) accumulators ([MM] holds M itself until the
+// frame's M' product replaces it), [NACC, +NBINQ) the bin terms, then the bin
+// index.  st: the replica's state; F: the frame's spin-derived values.  One
+// thread per replica; every term is written (zeros included).
+template <int NC>
+__device__ __forceinline__ void frame_terms(const mcg_graph& G, const mcg_batch& B, const double* st, double* F,
+                                            int flags, int64_t frame, double* Cn) {
+  const int ngq = frame_groups<NC>(G, flags);
+  for (int q = 0; q < MCG_NACC + MCG_NBINQ + 1; ++q) Cn[q] = 0.0;
+  const double n = G.n;
+  const double nLat = G.nlat > 0 ? G.nlat : G.n;
+  const double inv_nLat = 1.0 / nLat;
+  const bool pairs = (flags & MCG_MEAS_PAIRS) && G.nlat > 0;
+  double E = st[MCG_S_E];
+  if (flags & MCG_ISING_RELE) E -= st[MCG_S_EREF];
+  const double e = E * (1.0 / n);
+  const double M0 = st[MCG_S_M], M1 = st[MCG_S_M + 1], M2 = st[MCG_S_M + 2];
+  double Mf;
+  const int ax = NC == 3 ? 2 : 0;
+  if constexpr (NC == 1) {
+    Mf = (pairs ? F[0] : M0) * inv_nLat;
+    if (pairs) {
+      Cn[MCG_A_SI] = fabs(F[0]) * inv_nLat;
+      Cn[MCG_A_SJ] = fabs(F[3]) * inv_nLat;
+      Cn[MCG_A_SIJ] = F[6] * inv_nLat;
+      Cn[MCG_A_LAST] = F[7];
+    }
+    Cn[MCG_A_TOT] = M0;
+  } else {
+    const double Mv[3] = {M0, M1, NC == 3 ? M2 : 0.0};
+    double dir[3] = {Mv[0], Mv[1], Mv[2]};
+    const double ml = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    if (ml >= 1e-5) {
+      dir[0] /= ml; dir[1] /= ml; dir[2] /= ml;
+    }
+    if (NC == 3 || !pairs)
+      Mf = ml * inv_nLat;
+    else
+      Mf = sqrt(F[0] * F[0] + F[1] * F[1]) * inv_nLat;
+    if (pairs) {
+      for (int k = 0; k < NC; ++k) {
+        Cn[MCG_A_SI + k] = fabs(F[k] * inv_nLat);
+        Cn[MCG_A_SJ + k] = fabs(F[3 + k] * inv_nLat);
+      }
+      Cn[MCG_A_SIJ] = F[6] * inv_nLat;
+      Cn[MCG_A_ZI] = (dir[0] * F[0] + dir[1] * F[1] + dir[2] * F[2]) * inv_nLat;
+      Cn[MCG_A_ZJ] = (dir[0] * F[3] + dir[1] * F[4] + dir[2] * F[5]) * inv_nLat;
+      Cn[MCG_A_HI] = F[ax] * inv_nLat;
+      Cn[MCG_A_HJ] = F[3 + ax] * inv_nLat;
+    }
+    Cn[MCG_A_ZT] = (dir[0] * Mv[0] + dir[1] * Mv[1] + dir[2] * Mv[2]) * inv_nLat;
+    Cn[MCG_A_HT] = Mv[ax] * inv_nLat;
+    if ((flags & MCG_MEAS_Q) && NC == 3 && G.ncirc > 0) Cn[MCG_A_Q] = F[8] / REF_PI / 4;
+    if ((flags & MCG_MEAS_RNORM) && G.nr > 0) {
+      for (int k = 0; k < NC; ++k) {
+        Cn[MCG_A_SIR + k] = fabs(F[16 + k] / F[24]);
+        Cn[MCG_A_SJR + k] = fabs(F[19 + k] / F[25]);
+      }
+      Cn[MCG_A_SIJR] = F[22] / F[26];
+    }
+    // groups: the total (all sites / nLat) is group ng
+    if (ngq > 0)
+      for (int k = 0; k < 3; ++k) F[32 + 3 * ngq + k] = Mv[k] * inv_nLat;
+  }
+  if ((flags & MCG_MEAS_RNORM) && G.nr > 0) {
+    const double er = (F[27] / 2 + F[28]) / G.nr;
+    Cn[MCG_A_ER] = er;
+    Cn[MCG_A_ER2] = er * er;
+  }
+  Cn[MCG_A_E] = e;
+  Cn[MCG_A_E2] = e * e;
+  Cn[MCG_A_M] = Mf;
+  Cn[MCG_A_M2] = Mf * Mf;
+  Cn[MCG_A_M4] = Mf * Mf * Mf * Mf;
+  Cn[MCG_A_MM] = Mf;  // applied as A[MM] += A[MPREV] * Mf; A[MPREV] = Mf
+  Cn[MCG_A_COUNT] = 1.0;
+  Cn[MCG_A_ABSM] = fabs(Mf);
+  // bins for error bars: bin = floor(frame * nbins / nframes), from an FP64
+  // quotient corrected to the exact integer floor
+  const int64_t nf = B.nframes > 0 ? B.nframes : 1;
+  const int64_t num = frame * (int64_t)B.nbins;
+  int64_t bin = (int64_t)floor((double)num / (double)nf);
+  if (bin * nf > num) --bin;
+  if ((bin + 1) * nf <= num) ++bin;
+  if (bin >= B.nbins) bin = B.nbins - 1;
+  if (bin < 0) bin = 0;
+  Cn[MCG_NACC + 0] = e;
+  Cn[MCG_NACC + 1] = e * e;
+  Cn[MCG_NACC + 2] = Mf;
+  Cn[MCG_NACC + 3] = Mf * Mf;
+  Cn[MCG_NACC + 4] = Mf * Mf * Mf * Mf;
+  Cn[MCG_NACC + MCG_NBINQ] = (double)bin;
+}
+
 // One measured frame of the group's replica at `row`: the observables are
 // computed and staged in the group's frame area, <M M'> and M' are brought
 // up to date at once (thread MCG_A_MM; mprev, if have_prev, is M' read ahead
@@ -749,92 +845,7 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
   //     in parallel (one read-modify-write each instead of a serial chain)
   double* Cn = F + FRM_ACC;  // [0, NACC) accumulators, [NACC, +NBINQ) bin, then bin index
   const int ngq = frame_groups<NC>(G, flags);
-  if (lt == 0) {
-    for (int q = 0; q < MCG_NACC + MCG_NBINQ + 1; ++q) Cn[q] = 0.0;
-    const double n = G.n;
-    const double nLat = G.nlat > 0 ? G.nlat : G.n;
-    const double inv_nLat = 1.0 / nLat;
-    const bool pairs = (flags & MCG_MEAS_PAIRS) && G.nlat > 0;
-    double E = st[MCG_S_E];
-    if (flags & MCG_ISING_RELE) E -= st[MCG_S_EREF];
-    const double e = E * (1.0 / n);
-    const double M0 = st[MCG_S_M], M1 = st[MCG_S_M + 1], M2 = st[MCG_S_M + 2];
-    double Mf;
-    const int ax = NC == 3 ? 2 : 0;
-    if constexpr (NC == 1) {
-      Mf = (pairs ? F[0] : M0) * inv_nLat;
-      if (pairs) {
-        Cn[MCG_A_SI] = fabs(F[0]) * inv_nLat;
-        Cn[MCG_A_SJ] = fabs(F[3]) * inv_nLat;
-        Cn[MCG_A_SIJ] = F[6] * inv_nLat;
-        Cn[MCG_A_LAST] = F[7];
-      }
-      Cn[MCG_A_TOT] = M0;
-    } else {
-      const double Mv[3] = {M0, M1, NC == 3 ? M2 : 0.0};
-      double dir[3] = {Mv[0], Mv[1], Mv[2]};
-      const double ml = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
-      if (ml >= 1e-5) {
-        dir[0] /= ml; dir[1] /= ml; dir[2] /= ml;
-      }
-      if (NC == 3 || !pairs)
-        Mf = ml * inv_nLat;
-      else
-        Mf = sqrt(F[0] * F[0] + F[1] * F[1]) * inv_nLat;
-      if (pairs) {
-        for (int k = 0; k < NC; ++k) {
-          Cn[MCG_A_SI + k] = fabs(F[k] * inv_nLat);
-          Cn[MCG_A_SJ + k] = fabs(F[3 + k] * inv_nLat);
-        }
-        Cn[MCG_A_SIJ] = F[6] * inv_nLat;
-        Cn[MCG_A_ZI] = (dir[0] * F[0] + dir[1] * F[1] + dir[2] * F[2]) * inv_nLat;
-        Cn[MCG_A_ZJ] = (dir[0] * F[3] + dir[1] * F[4] + dir[2] * F[5]) * inv_nLat;
-        Cn[MCG_A_HI] = F[ax] * inv_nLat;
-        Cn[MCG_A_HJ] = F[3 + ax] * inv_nLat;
-      }
-      Cn[MCG_A_ZT] = (dir[0] * Mv[0] + dir[1] * Mv[1] + dir[2] * Mv[2]) * inv_nLat;
-      Cn[MCG_A_HT] = Mv[ax] * inv_nLat;
-      if ((flags & MCG_MEAS_Q) && NC == 3 && G.ncirc > 0) Cn[MCG_A_Q] = F[8] / REF_PI / 4;
-      if ((flags & MCG_MEAS_RNORM) && G.nr > 0) {
-        for (int k = 0; k < NC; ++k) {
-          Cn[MCG_A_SIR + k] = fabs(F[16 + k] / F[24]);
-          Cn[MCG_A_SJR + k] = fabs(F[19 + k] / F[25]);
-        }
-        Cn[MCG_A_SIJR] = F[22] / F[26];
-      }
-      // groups: the total (all sites / nLat) is group ng
-      if (ngq > 0)
-        for (int k = 0; k < 3; ++k) F[32 + 3 * ngq + k] = Mv[k] * inv_nLat;
-    }
-    if ((flags & MCG_MEAS_RNORM) && G.nr > 0) {
-      const double er = (F[27] / 2 + F[28]) / G.nr;
-      Cn[MCG_A_ER] = er;
-      Cn[MCG_A_ER2] = er * er;
-    }
-    Cn[MCG_A_E] = e;
-    Cn[MCG_A_E2] = e * e;
-    Cn[MCG_A_M] = Mf;
-    Cn[MCG_A_M2] = Mf * Mf;
-    Cn[MCG_A_M4] = Mf * Mf * Mf * Mf;
-    Cn[MCG_A_MM] = Mf;  // applied as A[MM] += A[MPREV] * Mf; A[MPREV] = Mf
-    Cn[MCG_A_COUNT] = 1.0;
-    Cn[MCG_A_ABSM] = fabs(Mf);
-    // bins for error bars: bin = floor(frame * nbins / nframes), from an FP64
-    // quotient corrected to the exact integer floor
-    const int64_t nf = B.nframes > 0 ? B.nframes : 1;
-    const int64_t num = frame * (int64_t)B.nbins;
-    int64_t bin = (int64_t)floor((double)num / (double)nf);
-    if (bin * nf > num) --bin;
-    if ((bin + 1) * nf <= num) ++bin;
-    if (bin >= B.nbins) bin = B.nbins - 1;
-    if (bin < 0) bin = 0;
-    Cn[MCG_NACC + 0] = e;
-    Cn[MCG_NACC + 1] = e * e;
-    Cn[MCG_NACC + 2] = Mf;
-    Cn[MCG_NACC + 3] = Mf * Mf;
-    Cn[MCG_NACC + 4] = Mf * Mf * Mf * Mf;
-    Cn[MCG_NACC + MCG_NBINQ] = (double)bin;
-  }
+  if (lt == 0) frame_terms<NC>(G, B, st, F, flags, frame, Cn);
   __syncthreads();
   // A row's accumulators are written by whichever replica holds the row, so
   // within one persistent launch by workgroups on different XCDs: the
@@ -1131,6 +1142,67 @@ __device__ __forceinline__ void tail_rest(const SV& sv, const mcg_graph& G, cons
   __syncthreads();
 }
 
+// FP64 value of lane src (any lane) in every lane
+__device__ __forceinline__ double bperm_f64(double x, int src) {
+  const int lo = __builtin_amdgcn_ds_bpermute(src << 2, __double2loint(x));
+  const int hi = __builtin_amdgcn_ds_bpermute(src << 2, __double2hiint(x));
+  return __hiloint2double(hi, lo);
+}
+
+// End of a persistent sweep for a workgroup of unsplit replicas (no full
+// recompute due): the state update, frame terms, M' and published energy of
+// every replica are done by the workgroup's first wave, lane g for group g,
+// so that this serial per-replica work runs once per workgroup instead of
+// once in every group.  The sums are group_sum's (each wave's DPP total,
+// then the same DPP steps over a group's waves), the frame terms those of
+// frame_terms; mprev_l, row_l, rep_l: lane g's group.  Called by every
+// thread; ends with a workgroup barrier.
+template <int NC>
+__device__ __forceinline__ void tail_wave0(const mcg_graph& G, const mcg_batch& B, double* s_red, double* s_st,
+                                           double* s_frm, int R, int TPR, float fE, const float* fM, bool meas,
+                                           bool therm_end, int64_t frame, int rep_l, int row_l, double mprev_l,
+                                           double* eslot_par) {
+  double v[4] = {(double)fE, (double)fM[0], (double)fM[1], (double)fM[2]};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = wave_sum_f64(v[k]);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 63)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s_red[wave * RK + k] = v[k];
+  __syncthreads();
+  if (wave == 0) {
+    const int nw = TPR >> 6, nwaves = (int)blockDim.x >> 6;
+    double t[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double x = lane < nwaves ? s_red[lane * RK + k] : 0.0;
+      if (nw >= 2) x += dpp_f64<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
+      if (nw >= 4) x += dpp_f64<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]
+      if (nw >= 8) x += dpp_f64<0x141, 0xF>(x);  // row_half_mirror
+      if (nw >= 16) x += dpp_f64<0x140, 0xF>(x); // row_mirror
+      t[k] = bperm_f64(x, lane * nw);
+    }
+    if (lane < R) {
+      double* st = s_st + lane * MCG_NSTATE;
+      st[MCG_S_E] += t[0];
+      st[MCG_S_M] += t[1];
+      st[MCG_S_M + 1] += t[2];
+      st[MCG_S_M + 2] += t[3];
+      if (therm_end) st[MCG_S_EREF] = st[MCG_S_E];
+      if (meas) {
+        double* F = s_frm + lane * FRM;
+        double* Cn = F + FRM_ACC;
+        frame_terms<NC>(G, B, st, F, B.meas & ~MEAS_SPIN_OBS, frame, Cn);
+        const double Mf = Cn[MCG_A_MM];
+        Cn[MCG_A_MM] = mprev_l * Mf;
+        dev_store(B.acc + (size_t)row_l * MCG_NACC + MCG_A_MPREV, Mf);
+      }
+      dev_store(eslot_par + (size_t)rep_l * MAX_PARTS, st[MCG_S_E]);
+    }
+  }
+  __syncthreads();
+}
+
 template <int NC, typename ST, bool FULL, typename SV>
 __device__ __forceinline__ void lds_sweep_tail(const SV& sv, const mcg_graph& G, const mcg_batch& B, const float* s_coup,
                                                const float* s_cls, double* s_red, double* st, FrameCtx& C, int g,
@@ -1337,12 +1409,12 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
     // point (double-buffered by sweep parity); split parts publish their
     // partial sums and the sites another part reads (export lists).  A grid
     // barrier follows whenever tempering is due or parts must exchange.
-    // Then every workgroup computes the swap decisions of every ladder
-    // itself, on its own copy of who/rung (identical in all workgroups:
-    // same energies, same Philox words), so no second barrier is needed:
-    // a workgroup can get at most one sweep ahead, and the parity buffers
-    // keep what the slower ones still read.  Workgroup 0 writes the tables
-    // back at the end.
+    // Then every workgroup computes the swap decisions of the ladders its
+    // replicas belong to itself, on its own copy of who/rung (identical in
+    // all workgroups that share a ladder: same energies, same Philox words),
+    // so no second barrier is needed: a workgroup can get at most one sweep
+    // ahead, and the parity buffers keep what the slower ones still read.
+    // Each ladder's owner writes its tables back at the end.
     typedef typename RawBits<sizeof(ST)>::T U;
     constexpr int GD = SitesPerDraw<NC>::G;
     const int nrow = B.nrow, nrepb = B.nrep;
@@ -1364,6 +1436,15 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
     const int DS = halo_desc_ints(G.ncol);
     int* s_flag = (int*)s_red;
     int nbar = 0;
+    // lane g of the first wave stands for group g in tail_wave0
+    const int lane = threadIdx.x & 63;
+    const bool l_grp = threadIdx.x < 64 && lane < R;
+    int rep_l = 0, row0_l = 0;
+    if (l_grp) {
+      const ItemRep irl = recs[w.off + lane];
+      rep_l = irl.rep;
+      row0_l = irl.row0;
+    }
     for (int s = 0; s < a.nsweeps; ++s) {
       const int64_t k = a.k_begin + s;
       const uint64_t kk = (uint64_t)(a.philox_base + k);
@@ -1371,34 +1452,42 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
       const int rowg = ir.row0 + s_rung[rep];
       const float bg = s_beta[rowg], hg = s_h[rowg];
       const bool meas = k >= a.nthermal && ((k - a.nthermal + 1) % a.spf) == 0;
-      // M' of the row, read ahead: the row's last frame was taken before the
-      // previous barrier (a split replica's frame is taken after it, so its
-      // part 0 reads M' in the frame itself)
-      double mprev = 0.0;
-      if (meas && pi.np == 1 && lt == MCG_A_MM) mprev = frame_mprev(B, rowg);
+      const bool therm_end = (k + 1 == a.nthermal);
+      const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
+      const bool pt_now = a.pt_every > 0 && ((k + 1) % a.pt_every) == 0;
+      // unsplit replicas without a full recompute due end the sweep in
+      // tail_wave0; M' of each row is read ahead there, as the row's last
+      // frame was taken before the previous barrier (a split replica's frame
+      // is taken after it, so its part 0 reads M' in the frame itself)
+      const bool fast = pi.np == 1 && !rec;
+      const int rowg_l = row0_l + s_rung[rep_l];
+      double mprev_l = 0.0;
+      if (meas && fast && l_grp) mprev_l = frame_mprev(B, rowg_l);
       float fE = 0.0f, fM[3] = {0.0f, 0.0f, 0.0f};
       sweep_colours<NC, ST, MODE>(G, sp, C.coup, C.clsdat, s_colptr, s_pid, s_pat, lt, TPR, bg, hg, kk, key0, key1, pi,
                                   fE, fM);
       double v[RK] = {(double)fE, (double)fM[0], (double)fM[1], (double)fM[2], 0.0, 0.0, 0.0, 0.0};
-      group_sum<4>(v, s_red, g, TPR);
-      const bool therm_end = (k + 1 == a.nthermal);
-      const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
-      const bool pt_now = a.pt_every > 0 && ((k + 1) % a.pt_every) == 0;
       double* es = a.eslot + ((size_t)par * nrepb + rep) * MAX_PARTS;
       double* ms = a.mslot + ((size_t)par * nrepb + rep) * MAX_PARTS * 3;
-      if (pi.np == 1) {
+      if (fast) {
+        // the frame's accumulator updates wait until after the barrier
+        // (flush below), so that the barrier does not wait for them
+        tail_wave0<NC>(G, B, s_red, s_st, s_frm, R, TPR, fE, fM, meas, therm_end,
+                       (k - a.nthermal + 1) / a.spf - 1 + a.frame0, rep_l, rowg_l, mprev_l,
+                       a.eslot + (size_t)par * nrepb * MAX_PARTS);
+      } else if (pi.np == 1) {
+        group_sum<4>(v, s_red, g, TPR);
         if (lt == 0) {
           st[MCG_S_E] += v[0];
           st[MCG_S_M] += v[1];
           st[MCG_S_M + 1] += v[2];
           st[MCG_S_M + 2] += v[3];
         }
-        // the frame's accumulator updates wait until after the barrier
-        // (flush below), so that the barrier does not wait for them
         tail_rest<NC, ST, false>(sp, G, B, C.coup, C.clsdat, s_red, st, C, g, lt, TPR, rowg, hg, k, a, meas, therm_end,
-                                 rec, key0, key1, mprev, true, true);
+                                 rec, key0, key1, 0.0, false, true);
         if (lt == 0) dev_store(es, st[MCG_S_E]);
       } else {
+        group_sum<4>(v, s_red, g, TPR);
         if (lt == 0) {
           dev_store(es + pi.part, (pi.part == 0 ? st[MCG_S_E] : 0.0) + v[0]);
           for (int q = 0; q < 3; ++q) dev_store(ms + pi.part * 3 + q, v[1 + q]);
