@@ -454,12 +454,13 @@ __device__ __forceinline__ double wave_sum_f64(double x) {
   return x;
 }
 
-// lane 63's value, in every lane of the wave
-__device__ __forceinline__ double lane63(double x) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(x), 63);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), 63);
+// lane src's value (src the same in every lane), in every lane of the wave
+__device__ __forceinline__ double lane_f64(double x, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), src);
   return __hiloint2double(hi, lo);
 }
+__device__ __forceinline__ double lane63(double x) { return lane_f64(x, 63); }
 
 // sum of N (<= RK) values over the TPR threads of group g (TPR a multiple of
 // 64); the result is valid in every lane of the group's first wave, so in
@@ -1162,19 +1163,20 @@ __device__ __forceinline__ void tail_wave0(const mcg_graph& G, const mcg_batch& 
                                            double* s_frm, int R, int TPR, float fE, const float* fM, bool meas,
                                            bool therm_end, int64_t frame, int rep_l, int row_l, double mprev_l,
                                            double* eslot_par) {
+  constexpr int NV = NC + 1;  // E and the spin components (the rest stay 0)
   double v[4] = {(double)fE, (double)fM[0], (double)fM[1], (double)fM[2]};
 #pragma unroll
-  for (int k = 0; k < 4; ++k) v[k] = wave_sum_f64(v[k]);
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum_f64(v[k]);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 63)
 #pragma unroll
-    for (int k = 0; k < 4; ++k) s_red[wave * RK + k] = v[k];
+    for (int k = 0; k < NV; ++k) s_red[wave * RK + k] = v[k];
   __syncthreads();
   if (wave == 0) {
     const int nw = TPR >> 6, nwaves = (int)blockDim.x >> 6;
-    double t[4];
+    double t[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < NV; ++k) {
       double x = lane < nwaves ? s_red[lane * RK + k] : 0.0;
       if (nw >= 2) x += dpp_f64<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
       if (nw >= 4) x += dpp_f64<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]
@@ -1184,10 +1186,8 @@ __device__ __forceinline__ void tail_wave0(const mcg_graph& G, const mcg_batch& 
     }
     if (lane < R) {
       double* st = s_st + lane * MCG_NSTATE;
-      st[MCG_S_E] += t[0];
-      st[MCG_S_M] += t[1];
-      st[MCG_S_M + 1] += t[2];
-      st[MCG_S_M + 2] += t[3];
+#pragma unroll
+      for (int k = 0; k < NV; ++k) st[MCG_S_E + k] += t[k];
       if (therm_end) st[MCG_S_EREF] = st[MCG_S_E];
       if (meas) {
         double* F = s_frm + lane * FRM;
@@ -1235,6 +1235,17 @@ __device__ __forceinline__ void sweep_colours(const mcg_graph& G, const Spins<ST
   constexpr int GD = SitesPerDraw<NC>::G;
   const int zmax = G.zmax;
   const bool split = pi.np > 1;
+  // A colour's groups are dealt out in rounds of TPR; the last, partial
+  // round falls to the first lanes of the deal.  The deal starts at wave r
+  // of the group, chosen so that the groups' partial rounds land on
+  // different SIMDs (wave w of a workgroup runs on SIMD w % 4).  Which
+  // thread takes a Philox group does not change its result.
+  {
+    const int nw = TPR >> 6, g = (int)threadIdx.x / TPR;
+    const int r = nw >= 4 ? g & 3 : nw == 2 ? (g >> 1) & 1 : 0;
+    lt = lt - 64 * r;
+    if (lt < 0) lt += TPR;
+  }
   for (int c = 0; c < G.ncol; ++c) {
     const int b = s_colptr[c], cnt = s_colptr[c + 1] - b;
     const int ngrp = (cnt + GD - 1) / GD;
@@ -1503,6 +1514,19 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
         ++nbar;
       }
       if (pi.np > 1) {
+        // part 0 (one replica, so its first wave is the group's) reads M'
+        // (lane 0) and part p's sums (lane p) first, so that their round
+        // trips overlap the import, then ends the sweep for the replica
+        const bool w0 = pi.part == 0 && threadIdx.x < 64;
+        const int lane = threadIdx.x & 63;
+        double mp = 0.0, pe = 0.0, pm[3] = {0.0, 0.0, 0.0};
+        if (w0) {
+          if (meas && lane == 0) mp = frame_mprev(B, rowg);
+          if (lane >= 1 && lane < pi.np) {
+            pe = dev_load(es + lane);
+            for (int q = 0; q < 3; ++q) pm[q] = dev_load(ms + lane * 3 + q);
+          }
+        }
         for (int p = 0; p < pi.np; ++p) {
           if (p == pi.part) continue;
           const int32_t* hq = pi.hx + (p - pi.part) * DS;
@@ -1513,18 +1537,29 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
             for (int q = 0; q < NC; ++q) lraw[(size_t)q * n + ex[i]] = dev_load(xi + (size_t)i * NC + q);
         }
         if (pi.part == 0) {
-          if (lt == 0) {
-            double E = st[MCG_S_E] + v[0];
-            double dm[3] = {v[1], v[2], v[3]};
+          if (w0) {
+            double E = lane_f64(v[0], 0), dm[3] = {lane_f64(v[1], 0), lane_f64(v[2], 0), lane_f64(v[3], 0)};
+            E = st[MCG_S_E] + E;
             for (int p = 1; p < pi.np; ++p) {
-              E += dev_load(es + p);
-              for (int q = 0; q < 3; ++q) dm[q] += dev_load(ms + p * 3 + q);
+              E += lane_f64(pe, p);
+              for (int q = 0; q < 3; ++q) dm[q] += lane_f64(pm[q], p);
             }
+          if (lane == 0) {
             st[MCG_S_E] = E;
             for (int q = 0; q < 3; ++q) st[MCG_S_M + q] += dm[q];
+            if (therm_end) st[MCG_S_EREF] = st[MCG_S_E];
+            if (meas) {
+              double* Cn = C.s_frm + FRM_ACC;
+              frame_terms<NC>(G, B, st, C.s_frm, B.meas & ~MEAS_SPIN_OBS,
+                              (k - a.nthermal + 1) / a.spf - 1 + a.frame0, Cn);
+              const double Mf = Cn[MCG_A_MM];
+              Cn[MCG_A_MM] = mp * Mf;
+              dev_store(B.acc + (size_t)rowg * MCG_NACC + MCG_A_MPREV, Mf);
+            }
           }
-          tail_rest<NC, ST, false>(sp, G, B, C.coup, C.clsdat, s_red, st, C, g, lt, TPR, rowg, hg, k, a, meas,
-                                   therm_end, false, key0, key1);
+          }
+          __syncthreads();
+          if (meas) flush_frame<NC>(C, B, rowg, B.meas & ~MEAS_SPIN_OBS);
         }
         __syncthreads();
       }
