@@ -45,18 +45,23 @@ HBM_PEAK = 8.0e12
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lds_xy_scan_hbm_traffic.md")
 
 
-def profiled_traffic():
-    """HBM bytes per launch of the XY-scan sweep kernel measured with rocprofv3
-    PMC counters (FETCH_SIZE, WRITE_SIZE, each scaled by a copy-kernel
-    calibration), read from the committed profile summary."""
+def profiled_traffic(sweeps_per_launch: float):
+    """HBM bytes of one launch of the XY-scan sweep kernel running the given
+    number of sweeps, from rocprofv3 PMC counters (FETCH_SIZE, WRITE_SIZE,
+    each scaled by a copy-kernel calibration) in the committed profile
+    summary: a fixed part per launch plus a part per sweep."""
+    vals = {}
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":")[1])
+                for key in ("traffic_fixed_bytes_per_launch:", "traffic_bytes_per_sweep:"):
+                    if line.startswith(key):
+                        vals[key] = float(line.split(":")[1])
     except OSError:
-        pass
-    return None
+        return None
+    if len(vals) != 2:
+        return None
+    return vals["traffic_fixed_bytes_per_launch:"] + sweeps_per_launch * vals["traffic_bytes_per_sweep:"]
 
 
 def _cpu_worker(args):
@@ -173,7 +178,8 @@ def main():
     k_ms, k_launch, k_sweeps, k_upd = timing[0], timing[1], timing[2], timing[3]
     achieved = (k_upd * bpu) / (k_ms / 1e3) if k_ms > 0 else 0.0
     roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": profiled_traffic(), "kernel": "k_lds<2,float>",
+            "frac": achieved / HBM_PEAK, "traffic": profiled_traffic(k_sweeps / max(k_launch, 1)),
+            "kernel": "k_lds<2,float>",
             "avg_launch_us": 1e3 * k_ms / max(k_launch, 1), "launches": int(k_launch),
             "algorithmic_bytes_per_launch": bpu * k_upd / max(k_launch, 1)}
     acc = b.accumulators()

@@ -157,8 +157,8 @@ typedef struct mcg_plan {
   int32_t frame0;          /* index of the first measured frame of this call */
   int32_t parts;           /* LDS kernel: workgroups per replica (0 = auto: split replicas of
                               8192+ sites that exceed one CU's share of the batch; 1 = never
-                              split; 2..4 = split every LDS replica).  A launch with split
-                              replicas runs one sweep. */
+                              split; 2..4 = split every LDS replica).  Without a persistent
+                              launch, a launch with split replicas runs one sweep. */
   int32_t no_persist;      /* nonzero: never use persistent launches of the LDS kernel (see
                               mcg_run); 0: use them when every workgroup can be resident */
   int64_t sweep0;          /* global sweep index of the first sweep (Philox counter) */
