@@ -24,6 +24,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -109,6 +110,10 @@ struct Spins {
   PT p;
   int n;
   __device__ __forceinline__ float get(int i, int c) const { return SIO<ST>::ld(p + (size_t)c * n + i); }
+  // the spin `off` bytes away from site i's (same component)
+  __device__ __forceinline__ float get_off(int i, int c, int off) const {
+    return SIO<ST>::ld((PT)((char*)(p + (size_t)c * n + i) + off));
+  }
   __device__ __forceinline__ void set(int i, int c, float v) const { SIO<ST>::st(p + (size_t)c * n + i, v); }
 };
 template <typename ST>
@@ -117,6 +122,7 @@ using GSpins = Spins<ST, GAS ST*>;
 // neighbour accessors: the global link table (site order) or a pattern row
 // (colour-permuted positions, LDS)
 struct GlobalLinks {
+  static constexpr bool BYTE_OFFSETS = false;
   const GAS uint32_t* lk;
   const GAS int32_t* lc;  // per-link coupling indices (wide graphs) or nullptr
   __device__ __forceinline__ int nbr(int l) const { return (int)(lk[l] & 0xFFFFFFu); }
@@ -131,6 +137,7 @@ struct GlobalLinks {
 // that four links come in one 128-bit read
 __host__ __device__ constexpr int pat_stride(int zmax) { return (zmax + 4) & ~3; }
 struct PatLinks {
+  static constexpr bool BYTE_OFFSETS = false;
   const int32_t* row;  // 16-byte aligned
   int p;
   __device__ __forceinline__ int nbr(int l) const { return p + ((int)((uint32_t)row[l] << 8) >> 8); }
@@ -142,6 +149,22 @@ struct PatLinks {
     j[1] = p + ((int)((uint32_t)w.y << 8) >> 8);
     j[2] = p + ((int)((uint32_t)w.z << 8) >> 8);
     j[3] = p + ((int)((uint32_t)w.w << 8) >> 8);
+  }
+};
+// pattern rows of a uniform-coupling graph in LDS: each link is the byte
+// offset from the site's spin to the neighbour's (same component), so that a
+// neighbour's address is one add
+struct PatOffLinks {
+  static constexpr bool BYTE_OFFSETS = true;
+  const int32_t* row;  // 16-byte aligned
+  int p;
+  __device__ __forceinline__ int off(int l) const { return row[l]; }
+  __device__ __forceinline__ void off4(int l, int* o) const {
+    const int4 w = *(const int4*)(row + l);
+    o[0] = w.x;
+    o[1] = w.y;
+    o[2] = w.z;
+    o[3] = w.w;
   }
 };
 
@@ -160,10 +183,12 @@ static constexpr double REF_PI = 3.1415926535;  // heisenbergLib.c:6
 // w_c = acceptance.
 // UNI: every link of the graph carries the tensor at coup (a uniform
 // graph); the field is that tensor times the sum of the neighbour spins.
+// aniso = false: no site class has an on-site term (D = 0), so the D terms,
+// each an exact +0 to dE, are skipped.
 template <int NC, typename SV, typename LK, typename CP, bool UNI = false>
 __device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, int zmax, CP coup, int ci, CP clsdat,
                                            float beta, float h, uint32_t w_a, uint32_t w_b, uint32_t w_c,
-                                           float& dEacc, float* dMacc) {
+                                           float& dEacc, float* dMacc, bool aniso = true) {
   const float S = clsdat[ci * 8 + 0];
   if constexpr (NC == 1) {
     float f = 0.0f;
@@ -175,11 +200,24 @@ __device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, in
       int l = 0;
       for (; l + 4 <= zmax; l += 4) {
         int j[4];
-        lk.nbr4(l, j);
-        const float a0 = sp.get(j[0], 0), a1 = sp.get(j[1], 0), a2 = sp.get(j[2], 0), a3 = sp.get(j[3], 0);
-        m += a0; m += a1; m += a2; m += a3;
+        float a[4];
+        if constexpr (LK::BYTE_OFFSETS) {
+          lk.off4(l, j);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) a[u] = sp.get_off(i, 0, j[u]);
+        } else {
+          lk.nbr4(l, j);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) a[u] = sp.get(j[u], 0);
+        }
+        m += a[0]; m += a[1]; m += a[2]; m += a[3];
       }
-      for (; l < zmax; ++l) m += sp.get(lk.nbr(l), 0);
+      for (; l < zmax; ++l) {
+        if constexpr (LK::BYTE_OFFSETS)
+          m += sp.get_off(i, 0, lk.off(l));
+        else
+          m += sp.get(lk.nbr(l), 0);
+      }
       f = coup[0] * m;
     } else {
       for (int l = 0; l < zmax; ++l) f = fmaf(coup[lk.cid(l) * 9], sp.get(lk.nbr(l), 0), f);
@@ -195,7 +233,6 @@ __device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, in
       dMacc[0] += -2.0f * s;
     }
   } else {
-    const float Dx = clsdat[ci * 8 + 1], Dy = clsdat[ci * 8 + 2], Dz = clsdat[ci * 8 + 3];
     const float s0 = sp.get(i, 0), s1 = sp.get(i, 1);
     const float s2 = NC == 3 ? sp.get(i, 2) : 0.0f;
     float H0 = 0.0f, H1 = 0.0f, H2 = 0.0f;
@@ -205,13 +242,23 @@ __device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, in
       int l = 0;
       for (; l + 4 <= zmax; l += 4) {
         int j[4];
-        lk.nbr4(l, j);
         float t0[4], t1[4], t2[4];
+        if constexpr (LK::BYTE_OFFSETS) {
+          lk.off4(l, j);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          t0[u] = sp.get(j[u], 0);
-          t1[u] = sp.get(j[u], 1);
-          t2[u] = NC == 3 ? sp.get(j[u], 2) : 0.0f;
+          for (int u = 0; u < 4; ++u) {
+            t0[u] = sp.get_off(i, 0, j[u]);
+            t1[u] = sp.get_off(i, 1, j[u]);
+            t2[u] = NC == 3 ? sp.get_off(i, 2, j[u]) : 0.0f;
+          }
+        } else {
+          lk.nbr4(l, j);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            t0[u] = sp.get(j[u], 0);
+            t1[u] = sp.get(j[u], 1);
+            t2[u] = NC == 3 ? sp.get(j[u], 2) : 0.0f;
+          }
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -221,10 +268,17 @@ __device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, in
         }
       }
       for (; l < zmax; ++l) {
-        const int j = lk.nbr(l);
-        S0 += sp.get(j, 0);
-        S1 += sp.get(j, 1);
-        if constexpr (NC == 3) S2 += sp.get(j, 2);
+        if constexpr (LK::BYTE_OFFSETS) {
+          const int o = lk.off(l);
+          S0 += sp.get_off(i, 0, o);
+          S1 += sp.get_off(i, 1, o);
+          if constexpr (NC == 3) S2 += sp.get_off(i, 2, o);
+        } else {
+          const int j = lk.nbr(l);
+          S0 += sp.get(j, 0);
+          S1 += sp.get(j, 1);
+          if constexpr (NC == 3) S2 += sp.get(j, 2);
+        }
       }
       H0 = coup[0] * S0; H0 = fmaf(coup[1], S1, H0);
       H1 = coup[3] * S0; H1 = fmaf(coup[4], S1, H1);
@@ -269,7 +323,8 @@ __device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, in
     dE = fmaf(d1, H1, dE);
     if constexpr (NC == 3) dE = fmaf(d2, H2, dE);
     float n0 = s0 + d0, n1 = s1 + d1, n2 = s2 + d2;
-    {
+    if (aniso) {
+      const float Dx = clsdat[ci * 8 + 1], Dy = clsdat[ci * 8 + 2], Dz = clsdat[ci * 8 + 3];
       const float a0 = n0 * n0, b0 = s0 * s0, q0 = a0 - b0;
       const float a1 = n1 * n1, b1 = s1 * s1, q1 = a1 - b1;
       dE = fmaf(Dx, q0, dE);
@@ -342,7 +397,7 @@ __device__ __forceinline__ void metro_group_pat(const Spins<ST>& sp, int b, int 
                                                 const uint8_t* pat, const int32_t* pattab, int zmax,
                                                 const float* coup, const float* clsdat, float beta, float h,
                                                 uint64_t sweep, uint32_t key0, uint32_t key1, float& dEacc,
-                                                float* dMacc) {
+                                                float* dMacc, bool aniso = true) {
   constexpr int G = SitesPerDraw<NC>::G;
   const u32x4 x = philox4x32_10(
       u32x4{(uint32_t)q, (uint32_t)sweep, (uint32_t)(sweep >> 32), TAG_METRO | ((uint32_t)colour << 8)}, key0, key1);
@@ -352,13 +407,15 @@ __device__ __forceinline__ void metro_group_pat(const Spins<ST>& sp, int b, int 
     const int lp = q * G + g;
     if (lp < cnt) {
       const int p = b + lp;
-      const int32_t* row = pattab + (int)pat[p] * pat_stride(zmax);
-      const PatLinks lk{row, p};
+      // 24-bit multiply: full rate, and pattern ids and strides are small
+      const int32_t* row = pattab + __umul24((uint32_t)pat[p], (uint32_t)pat_stride(zmax));
+      typedef typename std::conditional<UNI, PatOffLinks, PatLinks>::type LK;
+      const LK lk{row, p};
       const uint32_t wa = NC == 3 ? w[0] : 0u;
       const uint32_t wb = NC == 3 ? w[1] : NC == 2 ? w[2 * g] : 0u;
       const uint32_t wc = NC == 3 ? w[2] : NC == 2 ? w[2 * g + 1] : w[g];
-      metro_core<NC, Spins<ST>, PatLinks, const float*, UNI>(sp, p, lk, zmax, coup, row[zmax], clsdat, beta, h, wa, wb,
-                                                              wc, dEacc, dMacc);
+      metro_core<NC, Spins<ST>, LK, const float*, UNI>(sp, p, lk, zmax, coup, row[zmax], clsdat, beta, h, wa, wb, wc,
+                                                       dEacc, dMacc, aniso);
     }
   }
 }
@@ -1231,7 +1288,8 @@ template <int NC, typename ST, int MODE>
 __device__ __forceinline__ void sweep_colours(const mcg_graph& G, const Spins<ST>& sp, const float* coup,
                                               const float* clsdat, const int32_t* s_colptr, const uint8_t* s_pid,
                                               const int32_t* s_pat, int lt, int TPR, float beta, float h, uint64_t kk,
-                                              uint32_t key0, uint32_t key1, const PartInfo& pi, float& fE, float* fM) {
+                                              uint32_t key0, uint32_t key1, const PartInfo& pi, float& fE, float* fM,
+                                              bool aniso = true) {
   constexpr int GD = SitesPerDraw<NC>::G;
   const int zmax = G.zmax;
   const bool split = pi.np > 1;
@@ -1259,10 +1317,10 @@ __device__ __forceinline__ void sweep_colours(const mcg_graph& G, const Spins<ST
       const float* cp = UNI ? coup + G.ucid * 9 : coup;
       for (int q = q0 + lt; q < q1; q += TPR)
         metro_group_pat<NC, ST, UNI>(sp, b, cnt, q, c, s_pid, s_pat, zmax, cp, clsdat, beta, h, kk, key0, key1, fE,
-                                     fM);
+                                     fM, aniso);
       for (int j = lt; j < nx; j += TPR)
         metro_group_pat<NC, ST, UNI>(sp, b, cnt, xl[j], c, s_pid, s_pat, zmax, cp, clsdat, beta, h, kk, key0, key1,
-                                     xE, xM);
+                                     xE, xM, aniso);
     } else if constexpr (MODE == 1) {
       const GAS int32_t* lcid = G.lcid ? gas(G.lcid) : nullptr;
       if (G.ucid >= 0) {
@@ -1316,7 +1374,7 @@ __device__ __forceinline__ void lds_sweeps(const mcg_graph& G, const mcg_batch& 
                                            const float* clsdat, const int32_t* s_colptr, const uint8_t* s_pid,
                                            const int32_t* s_pat, double* s_red, double* st, FrameCtx& C, int g,
                                            int lt, int TPR, int row, float beta, float h, uint32_t key0,
-                                           uint32_t key1, const PartInfo& pi, double& dE, double* dM) {
+                                           uint32_t key1, const PartInfo& pi, double& dE, double* dM, bool aniso) {
   const bool split = pi.np > 1;
   dE = 0.0;
   dM[0] = dM[1] = dM[2] = 0.0;
@@ -1328,7 +1386,7 @@ __device__ __forceinline__ void lds_sweeps(const mcg_graph& G, const mcg_batch& 
     if (meas && !split && lt == MCG_A_MM) mprev = frame_mprev(B, row);
     float fE = 0.0f, fM[3] = {0.0f, 0.0f, 0.0f};  // this sweep's accepted changes
     sweep_colours<NC, ST, MODE>(G, sp, coup, clsdat, s_colptr, s_pid, s_pat, lt, TPR, beta, h, kk, key0, key1, pi, fE,
-                                fM);
+                                fM, aniso);
     dE += (double)fE;
     dM[0] += (double)fM[0];
     dM[1] += (double)fM[1];
@@ -1390,10 +1448,20 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
 
   for (int t = threadIdx.x; t < ncl * 9; t += blockDim.x) s_coup[t] = G.coup[t];
   for (int t = threadIdx.x; t < ncs * 8; t += blockDim.x) s_clsl[t] = G.clsdat[t];
+  // does any site class carry an on-site term? (metro_core skips them if not)
+  int anz = 0;
+  for (int t = threadIdx.x; t < G.ncls; t += blockDim.x)
+    anz |= (G.clsdat[t * 8 + 1] != 0.0f) | (G.clsdat[t * 8 + 2] != 0.0f) | (G.clsdat[t * 8 + 3] != 0.0f);
+  const bool aniso = __syncthreads_or(anz) != 0;
   for (int t = threadIdx.x; t <= G.ncol; t += blockDim.x) s_colptr[t] = G.col_ptr[t];
   if (perm) {
-    for (int t = threadIdx.x; t < G.npat * (zmax + 1); t += blockDim.x)
-      s_pat[(t / (zmax + 1)) * pat_stride(zmax) + t % (zmax + 1)] = G.pattab[t];
+    for (int t = threadIdx.x; t < G.npat * (zmax + 1); t += blockDim.x) {
+      const int l = t % (zmax + 1);
+      int32_t v = G.pattab[t];
+      // uniform graphs: links as byte offsets between spins (PatOffLinks)
+      if (MODE == 3 && l < zmax) v = ((int32_t)((uint32_t)v << 8) >> 8) * (int32_t)sizeof(ST);
+      s_pat[(t / (zmax + 1)) * pat_stride(zmax) + l] = v;
+    }
     for (int t = threadIdx.x; t < n; t += blockDim.x) s_pid[t] = G.pat[t];
   }
 
@@ -1476,7 +1544,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
       if (meas && fast && l_grp) mprev_l = frame_mprev(B, rowg_l);
       float fE = 0.0f, fM[3] = {0.0f, 0.0f, 0.0f};
       sweep_colours<NC, ST, MODE>(G, sp, C.coup, C.clsdat, s_colptr, s_pid, s_pat, lt, TPR, bg, hg, kk, key0, key1, pi,
-                                  fE, fM);
+                                  fE, fM, aniso);
       double v[RK] = {(double)fE, (double)fM[0], (double)fM[1], (double)fM[2], 0.0, 0.0, 0.0, 0.0};
       double* es = a.eslot + ((size_t)par * nrepb + rep) * MAX_PARTS;
       double* ms = a.mslot + ((size_t)par * nrepb + rep) * MAX_PARTS * 3;
@@ -1638,13 +1706,13 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   double dE, dM[3];
   if constexpr (MODE == 0 || MODE == 3)
     lds_sweeps<NC, ST, MODE, FULL>(G, B, a, sp, s_coup, s_clsl, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
-                                   beta, h, key0, key1, pi, dE, dM);
+                                   beta, h, key0, key1, pi, dE, dM, aniso);
   else if constexpr (MODE == 1)
     lds_sweeps<NC, ST, 1, FULL>(G, B, a, sp, s_coup, s_clsl, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
-                                beta, h, key0, key1, pi, dE, dM);
+                                beta, h, key0, key1, pi, dE, dM, aniso);
   else
     lds_sweeps<NC, ST, 2, FULL>(G, B, a, sp, G.coup, G.clsdat, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR,
-                                row, beta, h, key0, key1, pi, dE, dM);
+                                row, beta, h, key0, key1, pi, dE, dM, aniso);
   __syncthreads();
   if (pi.np == 1) {
     stage<ST, false>(my, gsp, NC * n, lt, TPR);
