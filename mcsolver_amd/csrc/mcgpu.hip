@@ -988,6 +988,7 @@ struct SweepArgs {
   double* eslot;             // [2][nrep][MAX_PARTS] published energies
   double* mslot;             // [2][nrep][MAX_PARTS][3] published magnetisation changes
   void* xbuf;                // [2][nslots][MAX_PARTS][max_export][NC] exported sites
+  double* mprev3;            // [3][nrow] M' of each row's last frames (frame f in slot f % 3)
   int32_t max_export;
   int32_t nslots;
 };
@@ -1207,19 +1208,32 @@ __device__ __forceinline__ double bperm_f64(double x, int src) {
   return __hiloint2double(hi, lo);
 }
 
-// End of a persistent sweep for a workgroup of unsplit replicas (no full
-// recompute due): the state update, frame terms, M' and published energy of
-// every replica are done by the workgroup's first wave, lane g for group g,
-// so that this serial per-replica work runs once per workgroup instead of
-// once in every group.  The sums are group_sum's (each wave's DPP total,
-// then the same DPP steps over a group's waves), the frame terms those of
-// frame_terms; mprev_l, row_l, rep_l: lane g's group.  Called by every
-// thread; ends with a workgroup barrier.
+// Frame f of the replica of lane g (persistent launches): its terms are
+// staged for flush_frame and its M' goes through slot f % 3 of a.mprev3
+// (the row's previous frame in slot (f-1) % 3).
 template <int NC>
-__device__ __forceinline__ void tail_wave0(const mcg_graph& G, const mcg_batch& B, double* s_red, double* s_st,
-                                           double* s_frm, int R, int TPR, float fE, const float* fM, bool meas,
-                                           bool therm_end, int64_t frame, int rep_l, int row_l, double mprev_l,
-                                           double* eslot_par) {
+__device__ __forceinline__ void take_frame(const mcg_graph& G, const mcg_batch& B, const SweepArgs& a, double* s_st,
+                                           double* s_frm, int lane, int64_t f, int row, int nrow) {
+  double* F = s_frm + lane * FRM;
+  double* Cn = F + FRM_ACC;
+  const double mp = dev_load(a.mprev3 + (size_t)((f + 2) % 3) * nrow + row);
+  frame_terms<NC>(G, B, s_st + lane * MCG_NSTATE, F, B.meas & ~MEAS_SPIN_OBS, f, Cn);
+  const double Mf = Cn[MCG_A_MM];
+  Cn[MCG_A_MM] = mp * Mf;
+  dev_store(a.mprev3 + (size_t)(f % 3) * nrow + row, Mf);
+}
+
+// End of a persistent sweep for a workgroup of unsplit replicas: the state
+// update (and published energy) of every replica is done by the
+// workgroup's first wave, lane g for group g, so that this serial
+// per-replica work runs once per workgroup instead of once in every group.
+// The sums are group_sum's (each wave's DPP total, then the same DPP steps
+// over a group's waves).  rep_l: lane g's replica; eslot_par: the energy
+// slots of this sweep's parity, or nullptr to publish nothing.  Called by
+// every thread; ends with a workgroup barrier.
+template <int NC>
+__device__ __forceinline__ void tail_wave0(const mcg_batch& B, double* s_red, double* s_st, int R, int TPR, float fE,
+                                           const float* fM, bool therm_end, int rep_l, double* eslot_par) {
   constexpr int NV = NC + 1;  // E and the spin components (the rest stay 0)
   double v[4] = {(double)fE, (double)fM[0], (double)fM[1], (double)fM[2]};
 #pragma unroll
@@ -1246,15 +1260,7 @@ __device__ __forceinline__ void tail_wave0(const mcg_graph& G, const mcg_batch& 
 #pragma unroll
       for (int k = 0; k < NV; ++k) st[MCG_S_E + k] += t[k];
       if (therm_end) st[MCG_S_EREF] = st[MCG_S_E];
-      if (meas) {
-        double* F = s_frm + lane * FRM;
-        double* Cn = F + FRM_ACC;
-        frame_terms<NC>(G, B, st, F, B.meas & ~MEAS_SPIN_OBS, frame, Cn);
-        const double Mf = Cn[MCG_A_MM];
-        Cn[MCG_A_MM] = mprev_l * Mf;
-        dev_store(B.acc + (size_t)row_l * MCG_NACC + MCG_A_MPREV, Mf);
-      }
-      dev_store(eslot_par + (size_t)rep_l * MAX_PARTS, st[MCG_S_E]);
+      if (eslot_par) dev_store(eslot_par + (size_t)rep_l * MAX_PARTS, st[MCG_S_E]);
     }
   }
   __syncthreads();
@@ -1515,14 +1521,31 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
     const int DS = halo_desc_ints(G.ncol);
     int* s_flag = (int*)s_red;
     int nbar = 0;
-    // lane g of the first wave stands for group g in tail_wave0
+    // Lane g of the first wave stands for group g (unsplit replicas) or for
+    // the replica (part 0 of a split one) in the end-of-sweep work.  M' of a
+    // row (for <M M'>) passes from frame to frame through slot f % 3 of
+    // a.mprev3: frame f of a row is taken after the barrier of its sweep,
+    // reads slot (f-1) % 3 and writes slot f % 3; the writer of frame f-1
+    // stored before it arrived at this barrier, and a workgroup is never more
+    // than one sweep ahead, so three slots never collide.  A[MPREV] seeds the
+    // slots of the rows held at the start and takes the last frame's M'.
     const int lane = threadIdx.x & 63;
-    const bool l_grp = threadIdx.x < 64 && lane < R;
-    int rep_l = 0, row0_l = 0;
+    const int nfr = pi.np == 1 ? R : pi.part == 0 ? 1 : 0;  // groups whose frames this workgroup takes
+    const bool l_grp = threadIdx.x < 64 && lane < nfr;
+    const int pt0 = blockDim.x >= 128 ? 64 : 0;  // tempering on the waves after the first
+    // Frames are taken after the barrier when every sweep has one (split
+    // replicas, tempering every sweep); otherwise before it, so that a
+    // barrier always separates two frames of a row taken by different
+    // workgroups.
+    const bool frames_after = a.has_split || a.pt_every == 1;
+    int rep_l = 0, row0_l = 0, last_row = -1;
     if (l_grp) {
       const ItemRep irl = recs[w.off + lane];
       rep_l = irl.rep;
       row0_l = irl.row0;
+      const int r = row0_l + s_rung[rep_l];
+      const double m0 = B.acc[(size_t)r * MCG_NACC + MCG_A_MPREV];
+      for (int q = 0; q < 3; ++q) dev_store(a.mprev3 + (size_t)q * nrow + r, m0);
     }
     for (int s = 0; s < a.nsweeps; ++s) {
       const int64_t k = a.k_begin + s;
@@ -1534,37 +1557,30 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
       const bool therm_end = (k + 1 == a.nthermal);
       const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
       const bool pt_now = a.pt_every > 0 && ((k + 1) % a.pt_every) == 0;
-      // unsplit replicas without a full recompute due end the sweep in
-      // tail_wave0; M' of each row is read ahead there, as the row's last
-      // frame was taken before the previous barrier (a split replica's frame
-      // is taken after it, so its part 0 reads M' in the frame itself)
-      const bool fast = pi.np == 1 && !rec;
       const int rowg_l = row0_l + s_rung[rep_l];
-      double mprev_l = 0.0;
-      if (meas && fast && l_grp) mprev_l = frame_mprev(B, rowg_l);
       float fE = 0.0f, fM[3] = {0.0f, 0.0f, 0.0f};
       sweep_colours<NC, ST, MODE>(G, sp, C.coup, C.clsdat, s_colptr, s_pid, s_pat, lt, TPR, bg, hg, kk, key0, key1, pi,
                                   fE, fM, aniso);
       double v[RK] = {(double)fE, (double)fM[0], (double)fM[1], (double)fM[2], 0.0, 0.0, 0.0, 0.0};
       double* es = a.eslot + ((size_t)par * nrepb + rep) * MAX_PARTS;
       double* ms = a.mslot + ((size_t)par * nrepb + rep) * MAX_PARTS * 3;
-      if (fast) {
-        // the frame's accumulator updates wait until after the barrier
-        // (flush below), so that the barrier does not wait for them
-        tail_wave0<NC>(G, B, s_red, s_st, s_frm, R, TPR, fE, fM, meas, therm_end,
-                       (k - a.nthermal + 1) / a.spf - 1 + a.frame0, rep_l, rowg_l, mprev_l,
-                       a.eslot + (size_t)par * nrepb * MAX_PARTS);
-      } else if (pi.np == 1) {
-        group_sum<4>(v, s_red, g, TPR);
-        if (lt == 0) {
-          st[MCG_S_E] += v[0];
-          st[MCG_S_M] += v[1];
-          st[MCG_S_M + 1] += v[2];
-          st[MCG_S_M + 2] += v[3];
+      if (pi.np == 1) {
+        // state update and published energy before the barrier (with a full
+        // recompute due, the recomputed state is published instead)
+        tail_wave0<NC>(B, s_red, s_st, R, TPR, fE, fM, therm_end && !rec, rep_l,
+                       rec ? nullptr : a.eslot + (size_t)par * nrepb * MAX_PARTS);
+        if (rec) {
+          double r4[4];
+          recompute<NC>(sp, G, C.coup, C.clsdat, hg, s_red, g, lt, TPR, r4);
+          if (lt == 0) {
+            const double d = fabs(st[MCG_S_E] - r4[0]);
+            if (d > st[MCG_S_DRIFT]) st[MCG_S_DRIFT] = d;
+            for (int q = 0; q < 4; ++q) st[MCG_S_E + q] = r4[q];
+            if (therm_end) st[MCG_S_EREF] = st[MCG_S_E];
+            dev_store(es, st[MCG_S_E]);
+          }
+          __syncthreads();
         }
-        tail_rest<NC, ST, false>(sp, G, B, C.coup, C.clsdat, s_red, st, C, g, lt, TPR, rowg, hg, k, a, meas, therm_end,
-                                 rec, key0, key1, 0.0, false, true);
-        if (lt == 0) dev_store(es, st[MCG_S_E]);
       } else {
         group_sum<4>(v, s_red, g, TPR);
         if (lt == 0) {
@@ -1577,23 +1593,21 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
         for (int i = lt; i < nex; i += TPR)
           for (int q = 0; q < NC; ++q) dev_store(xo + (size_t)i * NC + q, lraw[(size_t)q * n + ex[i]]);
       }
+      if (meas && l_grp && !frames_after) {
+        take_frame<NC>(G, B, a, s_st, s_frm, lane, (k - a.nthermal + 1) / a.spf - 1 + a.frame0, rowg_l, nrow);
+        last_row = rowg_l;
+      }
       if (a.has_split || pt_now) {
         if (!grid_barrier(a.bar, (uint32_t)(a.bar_base + nbar + 1), B.err, s_flag)) return;
         ++nbar;
       }
       if (pi.np > 1) {
-        // part 0 (one replica, so its first wave is the group's) reads M'
-        // (lane 0) and part p's sums (lane p) first, so that their round
-        // trips overlap the import, then ends the sweep for the replica
+        // part 0 reads part p's sums in lane p, overlapping the import
         const bool w0 = pi.part == 0 && threadIdx.x < 64;
-        const int lane = threadIdx.x & 63;
-        double mp = 0.0, pe = 0.0, pm[3] = {0.0, 0.0, 0.0};
-        if (w0) {
-          if (meas && lane == 0) mp = frame_mprev(B, rowg);
-          if (lane >= 1 && lane < pi.np) {
-            pe = dev_load(es + lane);
-            for (int q = 0; q < 3; ++q) pm[q] = dev_load(ms + lane * 3 + q);
-          }
+        double pe = 0.0, pm[3] = {0.0, 0.0, 0.0};
+        if (w0 && lane >= 1 && lane < pi.np) {
+          pe = dev_load(es + lane);
+          for (int q = 0; q < 3; ++q) pm[q] = dev_load(ms + lane * 3 + q);
         }
         for (int p = 0; p < pi.np; ++p) {
           if (p == pi.part) continue;
@@ -1604,32 +1618,24 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
           for (int i = lt; i < nex; i += TPR)
             for (int q = 0; q < NC; ++q) lraw[(size_t)q * n + ex[i]] = dev_load(xi + (size_t)i * NC + q);
         }
-        if (pi.part == 0) {
-          if (w0) {
-            double E = lane_f64(v[0], 0), dm[3] = {lane_f64(v[1], 0), lane_f64(v[2], 0), lane_f64(v[3], 0)};
-            E = st[MCG_S_E] + E;
-            for (int p = 1; p < pi.np; ++p) {
-              E += lane_f64(pe, p);
-              for (int q = 0; q < 3; ++q) dm[q] += lane_f64(pm[q], p);
-            }
+        if (w0) {
+          double E = st[MCG_S_E] + v[0], dm[3] = {v[1], v[2], v[3]};
+          for (int p = 1; p < pi.np; ++p) {
+            E += lane_f64(pe, p);
+            for (int q = 0; q < 3; ++q) dm[q] += lane_f64(pm[q], p);
+          }
           if (lane == 0) {
             st[MCG_S_E] = E;
             for (int q = 0; q < 3; ++q) st[MCG_S_M + q] += dm[q];
             if (therm_end) st[MCG_S_EREF] = st[MCG_S_E];
-            if (meas) {
-              double* Cn = C.s_frm + FRM_ACC;
-              frame_terms<NC>(G, B, st, C.s_frm, B.meas & ~MEAS_SPIN_OBS,
-                              (k - a.nthermal + 1) / a.spf - 1 + a.frame0, Cn);
-              const double Mf = Cn[MCG_A_MM];
-              Cn[MCG_A_MM] = mp * Mf;
-              dev_store(B.acc + (size_t)rowg * MCG_NACC + MCG_A_MPREV, Mf);
-            }
           }
-          }
-          __syncthreads();
-          if (meas) flush_frame<NC>(C, B, rowg, B.meas & ~MEAS_SPIN_OBS);
         }
-        __syncthreads();
+      }
+      // the frame of every replica (first wave, lane g), while the other
+      // waves run the tempering pass
+      if (meas && l_grp && frames_after) {
+        take_frame<NC>(G, B, a, s_st, s_frm, lane, (k - a.nthermal + 1) / a.spf - 1 + a.frame0, rowg_l, nrow);
+        last_row = rowg_l;
       }
       if (pt_now) {
         // the swaps of the ladders this workgroup's replicas belong to (the
@@ -1641,7 +1647,8 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
         const int L = a.blk_lad[li] & 0x7FFFFFFF;
         const int r0 = a.lad_row0[L];
         const int npairs = max(0, (a.lad_n[L] - pp) / 2);
-        for (int i = threadIdx.x; i < npairs; i += blockDim.x) {
+        for (int i = (int)threadIdx.x - pt0; i < npairs; i += (int)blockDim.x - pt0) {
+          if (i < 0) break;
           const int t = pp + 2 * i;
           const int ra = s_who[r0 + t], rb = s_who[r0 + t + 1];
           const double* ea = a.eslot + ((size_t)par * nrepb + ra) * MAX_PARTS;
@@ -1679,9 +1686,16 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
         }
         __syncthreads();
       }
-      if (meas && pi.np == 1) flush_frame<NC>(C, B, rowg, B.meas & ~MEAS_SPIN_OBS);
+      if (meas && nfr > 0) {
+        __syncthreads();
+        if (g < nfr) flush_frame<NC>(C, B, rowg, B.meas & ~MEAS_SPIN_OBS);
+      }
     }
     __syncthreads();
+    if (last_row >= 0) {
+      const int64_t fl = (a.k_begin + a.nsweeps - a.nthermal) / a.spf - 1 + a.frame0;  // last frame
+      B.acc[(size_t)last_row * MCG_NACC + MCG_A_MPREV] = dev_load(a.mprev3 + (size_t)(fl % 3) * nrow + last_row);
+    }
     if (pi.np == 1)
       stage<ST, false>(my, gsp, NC * n, lt, TPR);
     else
@@ -2150,7 +2164,7 @@ struct RunCtx {
   int nslots = 0;  // split replicas (arrival counters)
   int max_export = 0;  // longest export list of a part
   bool persist = false;  // persistent launches (barrier + tempering inside)
-  DevBuf bar, eslot, mslot, xbuf, rep_np, blk_lad;
+  DevBuf bar, eslot, mslot, xbuf, rep_np, blk_lad, mprev3;
   int mode_begin[5] = {0, 0, 0, 0, 0};  // work items of link mode m: [mode_begin[m], mode_begin[m+1])
   size_t lds_max = 0;
   std::vector<int32_t> hl_row0, hl_n;
@@ -2778,6 +2792,8 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
     const size_t xb = (size_t)2 * std::max(X.nslots, 1) * MAX_PARTS * std::max(X.max_export, 1) * NC * sizeof(ST);
     std::vector<unsigned char> zx(xb, 0);
     if (X.xbuf.upload(zx.data(), xb, s)) return -1;
+    std::vector<double> zm((size_t)3 * B->nrow, 0.0);
+    if (X.mprev3.upload(zm.data(), zm.size() * 8, s)) return -1;
     std::vector<uint8_t> np(B->nrep, 1);
     for (const WorkItem& w : X.hitems)
       if (w.nparts > 1) np[X.hitem_reps[w.off]] = w.nparts;
@@ -2830,6 +2846,7 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
       a.eslot = (double*)X.eslot.p;
       a.mslot = (double*)X.mslot.p;
       a.xbuf = X.xbuf.p;
+      a.mprev3 = (double*)X.mprev3.p;
       a.max_export = std::max(X.max_export, 1);
       a.nslots = std::max(X.nslots, 1);
       for (int64_t q = k; q < k + ns; ++q)
