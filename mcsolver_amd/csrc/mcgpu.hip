@@ -995,8 +995,11 @@ struct SweepArgs {
 
 // LDS of the tempering state a persistent launch keeps in every workgroup:
 // who and accepted-swap counts per row, rung per replica, beta and h per row
+// (and the workgroup's ladder list: at most one ladder per replica, so at
+// most MAX_LADW entries of (ladder | owner bit, first row, rungs))
+static constexpr int MAX_LADW = 16;
 __host__ __device__ inline size_t persist_lds_bytes(int nrow, int nrep) {
-  return align16((size_t)(2 * nrow + nrep) * 4) + align16((size_t)2 * nrow * 4);
+  return align16((size_t)(2 * nrow + nrep) * 4) + align16((size_t)2 * nrow * 4) + align16((size_t)MAX_LADW * 3 * 4);
 }
 
 static constexpr int FRM = 96 + 48;  // frame scratch doubles per group: scalars, then accumulator staging
@@ -1206,6 +1209,37 @@ __device__ __forceinline__ double bperm_f64(double x, int src) {
   const int lo = __builtin_amdgcn_ds_bpermute(src << 2, __double2loint(x));
   const int hi = __builtin_amdgcn_ds_bpermute(src << 2, __double2hiint(x));
   return __hiloint2double(hi, lo);
+}
+
+// One exported site of a split replica: its NC components (raw bits, SoA in
+// LDS) as one device-scope store when they fill 4 or 8 bytes, else one per
+// component; import_site is the matching load.
+template <int NC, typename U>
+__device__ __forceinline__ void export_site(U* dst, const U* lraw, int n, int site) {
+  constexpr int W = NC * (int)sizeof(U);
+  if constexpr (W == 8 || W == 4) {
+    typedef typename std::conditional<W == 8, uint64_t, uint32_t>::type P;
+    P w = 0;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) w |= (P)lraw[(size_t)q * n + site] << (q * 8 * sizeof(U));
+    dev_store((P*)dst, w);
+  } else {
+#pragma unroll
+    for (int q = 0; q < NC; ++q) dev_store(dst + q, lraw[(size_t)q * n + site]);
+  }
+}
+template <int NC, typename U>
+__device__ __forceinline__ void import_site(const U* src, U* lraw, int n, int site) {
+  constexpr int W = NC * (int)sizeof(U);
+  if constexpr (W == 8 || W == 4) {
+    typedef typename std::conditional<W == 8, uint64_t, uint32_t>::type P;
+    const P w = dev_load((const P*)src);
+#pragma unroll
+    for (int q = 0; q < NC; ++q) lraw[(size_t)q * n + site] = (U)(w >> (q * 8 * sizeof(U)));
+  } else {
+#pragma unroll
+    for (int q = 0; q < NC; ++q) lraw[(size_t)q * n + site] = dev_load(src + q);
+  }
 }
 
 // Frame f of the replica of lane g (persistent launches): its terms are
@@ -1508,6 +1542,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
     int32_t* s_rung = s_ptacc + nrow;
     float* s_beta = (float*)(s_pers + align16((size_t)(2 * nrow + nrepb) * 4));
     float* s_h = s_beta + nrow;
+    int32_t* s_lad = (int32_t*)(s_pers + align16((size_t)(2 * nrow + nrepb) * 4) + align16((size_t)2 * nrow * 4));
     for (int t = threadIdx.x; t < nrow; t += blockDim.x) {
       s_who[t] = B.who[t];
       s_ptacc[t] = 0;
@@ -1515,6 +1550,15 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
       s_h[t] = B.row_h[t];
     }
     for (int t = threadIdx.x; t < nrepb; t += blockDim.x) s_rung[t] = B.rung[t];
+    // this workgroup's ladders, kept in LDS (read every tempering pass)
+    const int nlad = min(a.blk_lad[blockIdx.x + 1] - a.blk_lad[blockIdx.x], MAX_LADW);
+    if ((int)threadIdx.x < nlad) {
+      const int e = a.blk_lad[a.blk_lad[blockIdx.x] + threadIdx.x];
+      const int L = e & 0x7FFFFFFF;
+      s_lad[3 * threadIdx.x] = e;
+      s_lad[3 * threadIdx.x + 1] = a.lad_row0[L];
+      s_lad[3 * threadIdx.x + 2] = a.lad_n[L];
+    }
     __syncthreads();
     U* lraw = (U*)my;
     U* xb = (U*)a.xbuf;
@@ -1582,7 +1626,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
           __syncthreads();
         }
       } else {
-        group_sum<4>(v, s_red, g, TPR);
+        group_sum<NC + 1>(v, s_red, g, TPR);
         if (lt == 0) {
           dev_store(es + pi.part, (pi.part == 0 ? st[MCG_S_E] : 0.0) + v[0]);
           for (int q = 0; q < 3; ++q) dev_store(ms + pi.part * 3 + q, v[1 + q]);
@@ -1591,7 +1635,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
         const int nex = pi.hx[2 * G.ncol + 1];
         U* xo = xb + (((size_t)par * a.nslots + w.slot) * MAX_PARTS + pi.part) * (size_t)a.max_export * NC;
         for (int i = lt; i < nex; i += TPR)
-          for (int q = 0; q < NC; ++q) dev_store(xo + (size_t)i * NC + q, lraw[(size_t)q * n + ex[i]]);
+          export_site<NC>(xo + (size_t)i * NC, lraw, n, ex[i]);
       }
       if (meas && l_grp && !frames_after) {
         take_frame<NC>(G, B, a, s_st, s_frm, lane, (k - a.nthermal + 1) / a.spf - 1 + a.frame0, rowg_l, nrow);
@@ -1616,7 +1660,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
           const int nex = hq[2 * G.ncol + 1];
           const U* xi = xb + (((size_t)par * a.nslots + w.slot) * MAX_PARTS + p) * (size_t)a.max_export * NC;
           for (int i = lt; i < nex; i += TPR)
-            for (int q = 0; q < NC; ++q) lraw[(size_t)q * n + ex[i]] = dev_load(xi + (size_t)i * NC + q);
+            import_site<NC>(xi + (size_t)i * NC, lraw, n, ex[i]);
         }
         if (w0) {
           double E = st[MCG_S_E] + v[0], dm[3] = {v[1], v[2], v[3]};
@@ -1642,11 +1686,10 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
         // other ladders' rows are never read here)
         const int64_t swap = a.philox_base + k;
         const int pp = (int)(swap & 1);
-        const int li0 = a.blk_lad[blockIdx.x], li1 = a.blk_lad[blockIdx.x + 1];
-        for (int li = li0; li < li1; ++li) {
-        const int L = a.blk_lad[li] & 0x7FFFFFFF;
-        const int r0 = a.lad_row0[L];
-        const int npairs = max(0, (a.lad_n[L] - pp) / 2);
+        for (int li = 0; li < nlad; ++li) {
+        const int L = s_lad[3 * li] & 0x7FFFFFFF;
+        const int r0 = s_lad[3 * li + 1];
+        const int npairs = max(0, (s_lad[3 * li + 2] - pp) / 2);
         for (int i = (int)threadIdx.x - pt0; i < npairs; i += (int)blockDim.x - pt0) {
           if (i < 0) break;
           const int t = pp + 2 * i;
@@ -1704,11 +1747,10 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
     // each ladder's state goes back from its owner (flag bit 31 of the
     // workgroup's ladder entry): the first workgroup holding one of its
     // replicas
-    for (int li = a.blk_lad[blockIdx.x]; li < a.blk_lad[blockIdx.x + 1]; ++li) {
-      if (!(a.blk_lad[li] & 0x80000000)) continue;
-      const int L = a.blk_lad[li] & 0x7FFFFFFF;
-      const int r0 = a.lad_row0[L];
-      for (int t = threadIdx.x; t < a.lad_n[L]; t += blockDim.x) {
+    for (int li = 0; li < nlad; ++li) {
+      if (!(s_lad[3 * li] & 0x80000000)) continue;
+      const int r0 = s_lad[3 * li + 1];
+      for (int t = threadIdx.x; t < s_lad[3 * li + 2]; t += blockDim.x) {
         const int r = s_who[r0 + t];
         B.who[r0 + t] = r;
         B.pt_acc[r0 + t] += s_ptacc[r0 + t];
@@ -2814,6 +2856,7 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
         if (L >= 0 && std::find(ls.begin(), ls.end(), L) == ls.end()) ls.push_back(L);
       }
       std::sort(ls.begin(), ls.end());
+      if (ls.size() > (size_t)MAX_LADW) return fail("internal: more than %d ladders in one workgroup", MAX_LADW);
       for (int32_t L : ls) {
         bl.push_back(owned[L] ? L : (int32_t)(L | 0x80000000u));
         owned[L] = 1;
