@@ -999,7 +999,8 @@ struct SweepArgs {
 // most MAX_LADW entries of (ladder | owner bit, first row, rungs))
 static constexpr int MAX_LADW = 16;
 __host__ __device__ inline size_t persist_lds_bytes(int nrow, int nrep) {
-  return align16((size_t)(2 * nrow + nrep) * 4) + align16((size_t)2 * nrow * 4) + align16((size_t)MAX_LADW * 3 * 4);
+  return align16((size_t)(2 * nrow + nrep) * 4) + align16((size_t)2 * nrow * 4) + align16((size_t)MAX_LADW * 3 * 4) +
+         align16((size_t)nrep);
 }
 
 static constexpr int FRM = 96 + 48;  // frame scratch doubles per group: scalars, then accumulator staging
@@ -1243,14 +1244,17 @@ __device__ __forceinline__ void import_site(const U* src, U* lraw, int n, int si
 }
 
 // Frame f of the replica of lane g (persistent launches): its terms are
-// staged for flush_frame and its M' goes through slot f % 3 of a.mprev3
-// (the row's previous frame in slot (f-1) % 3).
+// staged for flush_frame and its M' goes through slot f % 3 of a.mprev3;
+// mp is the row's previous M', slot (f-1) % 3 (frame_mprev3).
+template <int NC>
+__device__ __forceinline__ double frame_mprev3(const SweepArgs& a, int64_t f, int row, int nrow) {
+  return dev_load(a.mprev3 + (size_t)((f + 2) % 3) * nrow + row);
+}
 template <int NC>
 __device__ __forceinline__ void take_frame(const mcg_graph& G, const mcg_batch& B, const SweepArgs& a, double* s_st,
-                                           double* s_frm, int lane, int64_t f, int row, int nrow) {
+                                           double* s_frm, int lane, int64_t f, int row, int nrow, double mp) {
   double* F = s_frm + lane * FRM;
   double* Cn = F + FRM_ACC;
-  const double mp = dev_load(a.mprev3 + (size_t)((f + 2) % 3) * nrow + row);
   frame_terms<NC>(G, B, s_st + lane * MCG_NSTATE, F, B.meas & ~MEAS_SPIN_OBS, f, Cn);
   const double Mf = Cn[MCG_A_MM];
   Cn[MCG_A_MM] = mp * Mf;
@@ -1265,9 +1269,13 @@ __device__ __forceinline__ void take_frame(const mcg_graph& G, const mcg_batch& 
 // over a group's waves).  rep_l: lane g's replica; eslot_par: the energy
 // slots of this sweep's parity, or nullptr to publish nothing.  Called by
 // every thread; ends with a workgroup barrier.
+// The sweep's ΔE, ΔM of every group, in the first wave: lane g gets group
+// g's totals in t (the other waves get zeros).  The sums are group_sum's
+// (each wave's DPP total, then the same DPP steps over a group's waves).
+// Called by every thread; s_red is free again only after the caller's next
+// workgroup barrier.
 template <int NC>
-__device__ __forceinline__ void tail_wave0(const mcg_batch& B, double* s_red, double* s_st, int R, int TPR, float fE,
-                                           const float* fM, bool therm_end, int rep_l, double* eslot_par) {
+__device__ __forceinline__ void wave0_sums(double* s_red, int TPR, float fE, const float* fM, double* t) {
   constexpr int NV = NC + 1;  // E and the spin components (the rest stay 0)
   double v[4] = {(double)fE, (double)fM[0], (double)fM[1], (double)fM[2]};
 #pragma unroll
@@ -1277,9 +1285,9 @@ __device__ __forceinline__ void tail_wave0(const mcg_batch& B, double* s_red, do
 #pragma unroll
     for (int k = 0; k < NV; ++k) s_red[wave * RK + k] = v[k];
   __syncthreads();
+  t[0] = t[1] = t[2] = t[3] = 0.0;
   if (wave == 0) {
     const int nw = TPR >> 6, nwaves = (int)blockDim.x >> 6;
-    double t[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       double x = lane < nwaves ? s_red[lane * RK + k] : 0.0;
@@ -1289,6 +1297,17 @@ __device__ __forceinline__ void tail_wave0(const mcg_batch& B, double* s_red, do
       if (nw >= 16) x += dpp_f64<0x140, 0xF>(x); // row_mirror
       t[k] = bperm_f64(x, lane * nw);
     }
+  }
+}
+
+template <int NC>
+__device__ __forceinline__ void tail_wave0(const mcg_batch& B, double* s_red, double* s_st, int R, int TPR, float fE,
+                                           const float* fM, bool therm_end, int rep_l, double* eslot_par) {
+  constexpr int NV = NC + 1;
+  double t[4];
+  wave0_sums<NC>(s_red, TPR, fE, fM, t);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (wave == 0) {
     if (lane < R) {
       double* st = s_st + lane * MCG_NSTATE;
 #pragma unroll
@@ -1543,13 +1562,17 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
     float* s_beta = (float*)(s_pers + align16((size_t)(2 * nrow + nrepb) * 4));
     float* s_h = s_beta + nrow;
     int32_t* s_lad = (int32_t*)(s_pers + align16((size_t)(2 * nrow + nrepb) * 4) + align16((size_t)2 * nrow * 4));
+    uint8_t* s_np = (uint8_t*)s_lad + align16((size_t)MAX_LADW * 3 * 4);  // parts of each replica
     for (int t = threadIdx.x; t < nrow; t += blockDim.x) {
       s_who[t] = B.who[t];
       s_ptacc[t] = 0;
       s_beta[t] = B.row_beta[t];
       s_h[t] = B.row_h[t];
     }
-    for (int t = threadIdx.x; t < nrepb; t += blockDim.x) s_rung[t] = B.rung[t];
+    for (int t = threadIdx.x; t < nrepb; t += blockDim.x) {
+      s_rung[t] = B.rung[t];
+      s_np[t] = a.rep_np[t];
+    }
     // this workgroup's ladders, kept in LDS (read every tempering pass)
     const int nlad = min(a.blk_lad[blockIdx.x + 1] - a.blk_lad[blockIdx.x], MAX_LADW);
     if ((int)threadIdx.x < nlad) {
@@ -1626,8 +1649,9 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
           __syncthreads();
         }
       } else {
-        group_sum<NC + 1>(v, s_red, g, TPR);
-        if (lt == 0) {
+        // one replica: lane 0 of the first wave gets the part's sums
+        wave0_sums<NC>(s_red, TPR, fE, fM, v);
+        if (threadIdx.x == 0) {
           dev_store(es + pi.part, (pi.part == 0 ? st[MCG_S_E] : 0.0) + v[0]);
           for (int q = 0; q < 3; ++q) dev_store(ms + pi.part * 3 + q, v[1 + q]);
         }
@@ -1637,14 +1661,18 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
         for (int i = lt; i < nex; i += TPR)
           export_site<NC>(xo + (size_t)i * NC, lraw, n, ex[i]);
       }
+      const int64_t fcur = (k - a.nthermal + 1) / a.spf - 1 + a.frame0;
       if (meas && l_grp && !frames_after) {
-        take_frame<NC>(G, B, a, s_st, s_frm, lane, (k - a.nthermal + 1) / a.spf - 1 + a.frame0, rowg_l, nrow);
+        take_frame<NC>(G, B, a, s_st, s_frm, lane, fcur, rowg_l, nrow, frame_mprev3<NC>(a, fcur, rowg_l, nrow));
         last_row = rowg_l;
       }
       if (a.has_split || pt_now) {
         if (!grid_barrier(a.bar, (uint32_t)(a.bar_base + nbar + 1), B.err, s_flag)) return;
         ++nbar;
       }
+      // M' of the frame's row, read first so that it overlaps the import
+      double mp = 0.0;
+      if (meas && l_grp && frames_after) mp = frame_mprev3<NC>(a, fcur, rowg_l, nrow);
       if (pi.np > 1) {
         // part 0 reads part p's sums in lane p, overlapping the import
         const bool w0 = pi.part == 0 && threadIdx.x < 64;
@@ -1678,7 +1706,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
       // the frame of every replica (first wave, lane g), while the other
       // waves run the tempering pass
       if (meas && l_grp && frames_after) {
-        take_frame<NC>(G, B, a, s_st, s_frm, lane, (k - a.nthermal + 1) / a.spf - 1 + a.frame0, rowg_l, nrow);
+        take_frame<NC>(G, B, a, s_st, s_frm, lane, fcur, rowg_l, nrow, mp);
         last_row = rowg_l;
       }
       if (pt_now) {
@@ -1696,12 +1724,14 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
           const int ra = s_who[r0 + t], rb = s_who[r0 + t + 1];
           const double* ea = a.eslot + ((size_t)par * nrepb + ra) * MAX_PARTS;
           const double* eb = a.eslot + ((size_t)par * nrepb + rb) * MAX_PARTS;
-          // every part slot at once (slots past a replica's parts hold 0)
+          // the parts' slots at once (a replica's energy is the sum of its
+          // parts' slots, in part order)
+          const int npa = s_np[ra], npb = s_np[rb];
           double xa[MAX_PARTS], xb[MAX_PARTS];
 #pragma unroll
           for (int q = 0; q < MAX_PARTS; ++q) {
-            xa[q] = dev_load(ea + q);
-            xb[q] = dev_load(eb + q);
+            xa[q] = q < npa ? dev_load(ea + q) : 0.0;
+            xb[q] = q < npb ? dev_load(eb + q) : 0.0;
           }
           double Ea = xa[0], Eb = xb[0];
 #pragma unroll
