@@ -1042,7 +1042,7 @@ __host__ __device__ inline size_t lds_fixed_bytes(int ncoup, int ncls, int ncol,
                                                   int nwaves, int R) {
   size_t b = align16((size_t)(ncoup * 9 + ncls * 8) * 4) + align16((size_t)(ncol + 1) * 4);
   if (npat > 0) b += align16((size_t)npat * pat_stride(zmax) * 4) + align16((size_t)n);
-  return b + align16((size_t)nwaves * RK * 8) + align16((size_t)R * (MCG_NSTATE + FRM) * 8);
+  return b + align16((size_t)nwaves * RK * 8) + align16((size_t)R * (MCG_NSTATE + FRM) * 8) + align16((size_t)R * 16);
 }
 
 __host__ __device__ inline size_t lds_replica_bytes(int nc, int n, int ssize) {
@@ -1171,6 +1171,25 @@ __device__ void part_copy(ST* lds_s, ST* gsp, int n, int nc, const int32_t* colp
 // colour, then (start, count) of the export list
 __host__ __device__ inline int halo_desc_ints(int ncol) { return 2 * ncol + 2; }
 
+// The leftover groups of a workgroup whose replicas each have one wave
+// (64 threads per replica, R replicas, every colour with the same number of
+// Philox groups): each replica's last, partial round of rem groups goes to
+// a pool of R*rem groups dealt over the workgroup's first threads, so that
+// it costs ceil(R*rem/64) wave-rounds instead of R.  Thread t takes group
+// ngrp-rem + t%rem of replica t/rem (pr; -1: no pool group).  A group's
+// result does not depend on the thread that takes it; the ΔE, ΔM of the
+// pool groups are summed per replica in a fixed order (pool_sums).
+template <typename ST>
+struct PoolInfo {
+  int rem = 0;             // 0: no pool
+  int pr = -1, pq = 0;     // this thread's pool group: replica pr (group index), group ngrp-rem+pq
+  const float4* par = nullptr;  // per group: beta, h, key1 bits
+  ST* spin0 = nullptr;     // the workgroup's replicas in LDS, stride sstride
+  int sstride = 0;
+  double* pp = nullptr;    // pool partials [NC+1][nr], nr = R*rem
+  int nr = 0;
+};
+
 struct PartInfo {
   int part = 0, np = 1;            // this workgroup's part of the replica
   const int32_t* hx = nullptr;     // extra groups: [2c] start, [2c+1] count per colour, into halo
@@ -1269,6 +1288,19 @@ __device__ __forceinline__ void take_frame(const mcg_graph& G, const mcg_batch& 
 // over a group's waves).  rep_l: lane g's replica; eslot_par: the energy
 // slots of this sweep's parity, or nullptr to publish nothing.  Called by
 // every thread; ends with a workgroup barrier.
+// Pool partials (PoolInfo): thread t's ΔE, ΔM of its pool groups are kept
+// in pp[k*nr + t] (nr = R*rem, k < NC+1; pp is the frame area, free between
+// a sweep's frames and the next tail); pool_sums gives replica g the sum of
+// its rem entries in thread order.
+template <int NC>
+__device__ __forceinline__ void pool_sums(const double* pp, int nr, int rem, int g, double* out) {
+  for (int k = 0; k <= NC; ++k) {
+    double x = 0.0;
+    for (int i = 0; i < rem; ++i) x += pp[k * nr + g * rem + i];
+    out[k] = x;
+  }
+}
+
 // The sweep's ΔE, ΔM of every group, in the first wave: lane g gets group
 // g's totals in t (the other waves get zeros).  The sums are group_sum's
 // (each wave's DPP total, then the same DPP steps over a group's waves).
@@ -1302,7 +1334,8 @@ __device__ __forceinline__ void wave0_sums(double* s_red, int TPR, float fE, con
 
 template <int NC>
 __device__ __forceinline__ void tail_wave0(const mcg_batch& B, double* s_red, double* s_st, int R, int TPR, float fE,
-                                           const float* fM, bool therm_end, int rep_l, double* eslot_par) {
+                                           const float* fM, bool therm_end, int rep_l, double* eslot_par,
+                                           const double* pp, int prem) {
   constexpr int NV = NC + 1;
   double t[4];
   wave0_sums<NC>(s_red, TPR, fE, fM, t);
@@ -1310,6 +1343,12 @@ __device__ __forceinline__ void tail_wave0(const mcg_batch& B, double* s_red, do
   if (wave == 0) {
     if (lane < R) {
       double* st = s_st + lane * MCG_NSTATE;
+      if (prem > 0) {  // a replica's sweep change: its groups' sum plus its pool groups' sum
+        double ps[4];
+        pool_sums<NC>(pp, R * prem, prem, lane, ps);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) t[k] += ps[k];
+      }
 #pragma unroll
       for (int k = 0; k < NV; ++k) st[MCG_S_E + k] += t[k];
       if (therm_end) st[MCG_S_EREF] = st[MCG_S_E];
@@ -1324,16 +1363,22 @@ __device__ __forceinline__ void lds_sweep_tail(const SV& sv, const mcg_graph& G,
                                                const float* s_cls, double* s_red, double* st, FrameCtx& C, int g,
                                                int lt, int TPR, int row, float h, int64_t k, const SweepArgs& a,
                                                bool meas, bool therm_end, bool rec, double& dE, double* dM,
-                                               uint32_t key0, uint32_t key1, double mprev = 0.0,
-                                               bool have_prev = false) {
+                                               uint32_t key0, uint32_t key1, double mprev, bool have_prev,
+                                               const PoolInfo<ST>& pl) {
   double v[RK] = {dE, dM[0], dM[1], dM[2], 0, 0, 0, 0};
   group_sum<4>(v, s_red, g, TPR);
   if (lt == 0) {
+    if (pl.rem > 0) {
+      double ps[4];
+      pool_sums<NC>(pl.pp, pl.nr, pl.rem, g, ps);
+      for (int k = 0; k <= NC; ++k) v[k] += ps[k];
+    }
     st[MCG_S_E] += v[0];
     st[MCG_S_M] += v[1];
     st[MCG_S_M + 1] += v[2];
     st[MCG_S_M + 2] += v[3];
   }
+
   dE = 0.0;
   dM[0] = dM[1] = dM[2] = 0.0;
   tail_rest<NC, ST, FULL>(sv, G, B, s_coup, s_cls, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end, rec, key0,
@@ -1348,7 +1393,7 @@ __device__ __forceinline__ void sweep_colours(const mcg_graph& G, const Spins<ST
                                               const float* clsdat, const int32_t* s_colptr, const uint8_t* s_pid,
                                               const int32_t* s_pat, int lt, int TPR, float beta, float h, uint64_t kk,
                                               uint32_t key0, uint32_t key1, const PartInfo& pi, float& fE, float* fM,
-                                              bool aniso = true) {
+                                              bool aniso, const PoolInfo<ST>& pl, bool pfresh) {
   constexpr int GD = SitesPerDraw<NC>::G;
   const int zmax = G.zmax;
   const bool split = pi.np > 1;
@@ -1374,9 +1419,23 @@ __device__ __forceinline__ void sweep_colours(const mcg_graph& G, const Spins<ST
     if constexpr (MODE == 0 || MODE == 3) {
       constexpr bool UNI = MODE == 3;
       const float* cp = UNI ? coup + G.ucid * 9 : coup;
-      for (int q = q0 + lt; q < q1; q += TPR)
+      const int qm = q1 - pl.rem;  // the pool takes the last rem groups (unsplit replicas only)
+      for (int q = q0 + lt; q < qm; q += TPR)
         metro_group_pat<NC, ST, UNI>(sp, b, cnt, q, c, s_pid, s_pat, zmax, cp, clsdat, beta, h, kk, key0, key1, fE,
                                      fM, aniso);
+      if (pl.pr >= 0) {
+        const float4 pv = pl.par[pl.pr];
+        float pe = 0.0f, pm[3] = {0.0f, 0.0f, 0.0f};
+        const Spins<ST> psp{pl.spin0 + (size_t)pl.sstride * pl.pr, G.n};
+        metro_group_pat<NC, ST, UNI>(psp, b, cnt, qm + pl.pq, c, s_pid, s_pat, zmax, cp, clsdat, pv.x, pv.y, kk, key0,
+                                     __float_as_uint(pv.z), pe, pm, aniso);
+        // the first pool group after a tail stores, later ones add
+        double* q = pl.pp + threadIdx.x;
+        const bool first = pfresh && c == 0;
+        q[0] = first ? (double)pe : q[0] + (double)pe;
+#pragma unroll
+        for (int k = 1; k <= NC; ++k) q[k * pl.nr] = first ? (double)pm[k - 1] : q[k * pl.nr] + (double)pm[k - 1];
+      }
       for (int j = lt; j < nx; j += TPR)
         metro_group_pat<NC, ST, UNI>(sp, b, cnt, xl[j], c, s_pid, s_pat, zmax, cp, clsdat, beta, h, kk, key0, key1,
                                      xE, xM, aniso);
@@ -1433,10 +1492,12 @@ __device__ __forceinline__ void lds_sweeps(const mcg_graph& G, const mcg_batch& 
                                            const float* clsdat, const int32_t* s_colptr, const uint8_t* s_pid,
                                            const int32_t* s_pat, double* s_red, double* st, FrameCtx& C, int g,
                                            int lt, int TPR, int row, float beta, float h, uint32_t key0,
-                                           uint32_t key1, const PartInfo& pi, double& dE, double* dM, bool aniso) {
+                                           uint32_t key1, const PartInfo& pi, double& dE, double* dM, bool aniso,
+                                           const PoolInfo<ST>& pl) {
   const bool split = pi.np > 1;
   dE = 0.0;
   dM[0] = dM[1] = dM[2] = 0.0;
+  bool pfresh = true;  // the next pool group starts the pool partials
   for (int s = 0; s < a.nsweeps; ++s) {
     const int64_t k = a.k_begin + s;
     const uint64_t kk = (uint64_t)(a.philox_base + k);
@@ -1445,7 +1506,8 @@ __device__ __forceinline__ void lds_sweeps(const mcg_graph& G, const mcg_batch& 
     if (meas && !split && lt == MCG_A_MM) mprev = frame_mprev(B, row);
     float fE = 0.0f, fM[3] = {0.0f, 0.0f, 0.0f};  // this sweep's accepted changes
     sweep_colours<NC, ST, MODE>(G, sp, coup, clsdat, s_colptr, s_pid, s_pat, lt, TPR, beta, h, kk, key0, key1, pi, fE,
-                                fM, aniso);
+                                fM, aniso, pl, pfresh);
+    pfresh = false;
     dE += (double)fE;
     dM[0] += (double)fM[0];
     dM[1] += (double)fM[1];
@@ -1455,7 +1517,8 @@ __device__ __forceinline__ void lds_sweeps(const mcg_graph& G, const mcg_batch& 
     const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
     if (meas || therm_end || rec || s == a.nsweeps - 1) {
       lds_sweep_tail<NC, ST, FULL>(sp, G, B, coup, clsdat, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end,
-                                   rec, dE, dM, key0, key1, mprev, true);
+                                   rec, dE, dM, key0, key1, mprev, true, pl);
+      pfresh = true;
     }
   }
 }
@@ -1501,6 +1564,8 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   double* s_st = (double*)(smem + off);  // R * NSTATE
   double* s_frm = s_st + R * MCG_NSTATE;  // R * FRM
   off += align16((size_t)R * (MCG_NSTATE + FRM) * 8);
+  float4* s_par = (float4*)(smem + off);  // per group: beta, h, key1 bits (the pool's parameters)
+  off += align16((size_t)R * 16);
   unsigned char* s_pers = smem + off;  // tempering state of a persistent launch
   if (PERSIST) off += persist_lds_bytes(B.nrow, B.nrep);
   ST* s_spin = (ST*)(smem + off);
@@ -1529,10 +1594,33 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   stage<ST, true>(my, gsp, NC * n, lt, TPR);
   double* st = s_st + g * MCG_NSTATE;
   if (lt < MCG_NSTATE) st[lt] = B.state[(size_t)rep * MCG_NSTATE + lt];
+  if (lt == 0) s_par[g] = make_float4(beta, h, __uint_as_float(B.seed_hi ^ (uint32_t)ir.serial), 0.0f);
   __syncthreads();
 
   const uint32_t key0 = B.seed_lo, key1 = B.seed_hi ^ (uint32_t)ir.serial;
   const Spins<ST> sp{my, n};
+  // pool of the replicas' partial last rounds (PoolInfo)
+  PoolInfo<ST> pl;
+  if (perm && TPR == 64 && R > 1 && w.nparts <= 1) {
+    constexpr int GDp = SitesPerDraw<NC>::G;
+    int rem = -1;
+    for (int c = 0; c < G.ncol; ++c) {
+      const int rc = ((s_colptr[c + 1] - s_colptr[c] + GDp - 1) / GDp) % 64;
+      rem = rem < 0 ? rc : rem == rc ? rem : 0;
+    }
+    if (rem > 0 && rem * 4 <= FRM) pl.rem = rem;  // partials fit the frame area
+  }
+  if (pl.rem > 0) {
+    pl.par = s_par;
+    pl.spin0 = s_spin;
+    pl.sstride = (int)(lds_replica_bytes(NC, n, sizeof(ST)) / sizeof(ST));
+    pl.pp = s_frm;
+    pl.nr = R * pl.rem;
+    if ((int)threadIdx.x < R * pl.rem) {
+      pl.pr = (int)threadIdx.x / pl.rem;
+      pl.pq = (int)threadIdx.x - pl.pr * pl.rem;
+    }
+  }
   PartInfo pi;
   if (w.nparts > 1) {
     pi.part = w.part;
@@ -1625,9 +1713,13 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
       const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
       const bool pt_now = a.pt_every > 0 && ((k + 1) % a.pt_every) == 0;
       const int rowg_l = row0_l + s_rung[rep_l];
+      if (pl.rem > 0) {  // the pool's parameters of this sweep (rows move with tempering)
+        if (lt == 0) s_par[g] = make_float4(bg, hg, __uint_as_float(key1), 0.0f);
+        __syncthreads();
+      }
       float fE = 0.0f, fM[3] = {0.0f, 0.0f, 0.0f};
       sweep_colours<NC, ST, MODE>(G, sp, C.coup, C.clsdat, s_colptr, s_pid, s_pat, lt, TPR, bg, hg, kk, key0, key1, pi,
-                                  fE, fM, aniso);
+                                  fE, fM, aniso, pl, true);
       double v[RK] = {(double)fE, (double)fM[0], (double)fM[1], (double)fM[2], 0.0, 0.0, 0.0, 0.0};
       double* es = a.eslot + ((size_t)par * nrepb + rep) * MAX_PARTS;
       double* ms = a.mslot + ((size_t)par * nrepb + rep) * MAX_PARTS * 3;
@@ -1635,7 +1727,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
         // state update and published energy before the barrier (with a full
         // recompute due, the recomputed state is published instead)
         tail_wave0<NC>(B, s_red, s_st, R, TPR, fE, fM, therm_end && !rec, rep_l,
-                       rec ? nullptr : a.eslot + (size_t)par * nrepb * MAX_PARTS);
+                       rec ? nullptr : a.eslot + (size_t)par * nrepb * MAX_PARTS, s_frm, pl.rem);
         if (rec) {
           double r4[4];
           recompute<NC>(sp, G, C.coup, C.clsdat, hg, s_red, g, lt, TPR, r4);
@@ -1792,13 +1884,13 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   double dE, dM[3];
   if constexpr (MODE == 0 || MODE == 3)
     lds_sweeps<NC, ST, MODE, FULL>(G, B, a, sp, s_coup, s_clsl, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
-                                   beta, h, key0, key1, pi, dE, dM, aniso);
+                                   beta, h, key0, key1, pi, dE, dM, aniso, pl);
   else if constexpr (MODE == 1)
     lds_sweeps<NC, ST, 1, FULL>(G, B, a, sp, s_coup, s_clsl, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR, row,
-                                beta, h, key0, key1, pi, dE, dM, aniso);
+                                beta, h, key0, key1, pi, dE, dM, aniso, pl);
   else
     lds_sweeps<NC, ST, 2, FULL>(G, B, a, sp, G.coup, G.clsdat, s_colptr, s_pid, s_pat, s_red, st, C, g, lt, TPR,
-                                row, beta, h, key0, key1, pi, dE, dM, aniso);
+                                row, beta, h, key0, key1, pi, dE, dM, aniso, pl);
   __syncthreads();
   if (pi.np == 1) {
     stage<ST, false>(my, gsp, NC * n, lt, TPR);
@@ -1847,7 +1939,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
     }
   __syncthreads();
   lds_sweep_tail<NC, ST, FULL>(sp, G, B, C.coup, C.clsdat, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end,
-                               rec, tE, tM, key0, key1);
+                               rec, tE, tM, key0, key1, 0.0, false, PoolInfo<ST>{});
   if (lt < MCG_NSTATE) B.state[(size_t)rep * MCG_NSTATE + lt] = st[lt];
   }
 }

@@ -69,6 +69,8 @@ CASES = {
     "ising_2orb": (1, lambda: __import__("cases")._ising_2orb(8), [0.3, 0.5], 0.2),
     "xy_sq16": (2, lambda: lat.square(16, J=-1.0, D=(0.1, -0.05, 0.0)), [0.5, 1.1, 2.0, 3.0], 0.1),
     "xy_sq12_noD": (2, lambda: lat.square(12, J=-1.0), [0.7, 1.4], 0.0),
+    # 144 groups per colour: each replica's partial last round goes to the pool
+    "xy_sq24_pool": (2, lambda: lat.square(24, J=-1.0), [0.6, 0.9, 1.2, 2.0], 0.05),
     "heis_cubic4_noD": (3, lambda: lat.cubic(4, J=-1.0), [0.6, 1.6], 0.05),
     "heis_honeycomb": (3, lambda: _honeycomb(8, 3), [0.7, 1.5], 0.2),
     "heis_tri_dmi": (3, lambda: _triangular_dmi(12), [1.0, 4.0], 0.3),
