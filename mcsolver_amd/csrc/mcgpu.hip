@@ -701,7 +701,8 @@ __device__ __forceinline__ void frame_terms(const mcg_graph& G, const mcg_batch&
     double dir[3] = {Mv[0], Mv[1], Mv[2]};
     const double ml = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
     if (ml >= 1e-5) {
-      dir[0] /= ml; dir[1] /= ml; dir[2] /= ml;
+      dir[0] /= ml; dir[1] /= ml;
+      if constexpr (NC == 3) dir[2] /= ml;  // XY: 0 / ml = 0
     }
     if (NC == 3 || !pairs)
       Mf = ml * inv_nLat;
@@ -749,7 +750,7 @@ __device__ __forceinline__ void frame_terms(const mcg_graph& G, const mcg_batch&
   // quotient corrected to the exact integer floor
   const int64_t nf = B.nframes > 0 ? B.nframes : 1;
   const int64_t num = frame * (int64_t)B.nbins;
-  int64_t bin = (int64_t)floor((double)num / (double)nf);
+  int64_t bin = (int64_t)floor((double)num * (1.0 / (double)nf));  // corrected to the exact floor below
   if (bin * nf > num) --bin;
   if ((bin + 1) * nf <= num) ++bin;
   if (bin >= B.nbins) bin = B.nbins - 1;
@@ -1096,7 +1097,8 @@ __device__ __forceinline__ void stores_done() {
 static constexpr int BAR_GROUPS = 16;
 static constexpr int BAR_LINE = 32;  // uint32 per line
 static constexpr int BAR_WORDS = (1 + 2 * BAR_GROUPS) * BAR_LINE;
-__device__ __forceinline__ bool grid_barrier(uint32_t* bar, uint32_t epoch, int32_t* err, int* s_flag) {
+// (grid_arrive then grid_wait: work placed between them overlaps the wait)
+__device__ __forceinline__ void grid_arrive(uint32_t* bar, uint32_t epoch) {
   stores_done();
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1105,13 +1107,17 @@ __device__ __forceinline__ bool grid_barrier(uint32_t* bar, uint32_t epoch, int3
     const int ngrp = nb < BAR_GROUPS ? nb : BAR_GROUPS;
     const uint32_t gsz = (uint32_t)((nb - grp + BAR_GROUPS - 1) / BAR_GROUPS);
     uint32_t* gc = bar + (1 + grp) * BAR_LINE;
-    uint32_t* rel = bar + (1 + BAR_GROUPS + grp) * BAR_LINE;
     const uint32_t old = __hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (old + 1 == epoch * gsz) {
       const uint32_t o2 = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (o2 + 1 == epoch * (uint32_t)ngrp)
         for (int q = 0; q < ngrp; ++q) dev_store(bar + (1 + BAR_GROUPS + q) * BAR_LINE, epoch);
     }
+  }
+}
+__device__ __forceinline__ bool grid_wait(uint32_t* bar, uint32_t epoch, int32_t* err, int* s_flag) {
+  if (threadIdx.x == 0) {
+    uint32_t* rel = bar + (1 + BAR_GROUPS + (int)blockIdx.x % BAR_GROUPS) * BAR_LINE;
     int ok = 1;
     uint32_t it = 0;
     while ((int32_t)(dev_load(rel) - epoch) < 0) {
@@ -1263,8 +1269,8 @@ __device__ __forceinline__ void import_site(const U* src, U* lraw, int n, int si
 }
 
 // Frame f of the replica of lane g (persistent launches): its terms are
-// staged for flush_frame and its M' goes through slot f % 3 of a.mprev3;
-// mp is the row's previous M', slot (f-1) % 3 (frame_mprev3).
+// staged for flush_frame; mp is the row's previous M', slot (f-1) % 3 of
+// a.mprev3 (frame_mprev3), and the frame's own M goes to slot f % 3.
 template <int NC>
 __device__ __forceinline__ double frame_mprev3(const SweepArgs& a, int64_t f, int row, int nrow) {
   return dev_load(a.mprev3 + (size_t)((f + 2) % 3) * nrow + row);
@@ -1341,14 +1347,18 @@ __device__ __forceinline__ void tail_wave0(const mcg_batch& B, double* s_red, do
   wave0_sums<NC>(s_red, TPR, fE, fM, t);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (wave == 0) {
+    if (prem > 0) {
+      // a replica's sweep change: its groups' sum plus its pool groups' sum
+      // (pool_sums' order); lane 16k+g adds term k of replica g, R <= 16
+      const int gk = lane & 15, kk = lane >> 4;
+      double x = 0.0;
+      if (gk < R && kk < NV)
+        for (int i = 0; i < prem; ++i) x += pp[kk * R * prem + gk * prem + i];
+#pragma unroll
+      for (int k = 0; k < NV; ++k) t[k] += bperm_f64(x, 16 * k + (lane & 15));
+    }
     if (lane < R) {
       double* st = s_st + lane * MCG_NSTATE;
-      if (prem > 0) {  // a replica's sweep change: its groups' sum plus its pool groups' sum
-        double ps[4];
-        pool_sums<NC>(pp, R * prem, prem, lane, ps);
-#pragma unroll
-        for (int k = 0; k < NV; ++k) t[k] += ps[k];
-      }
 #pragma unroll
       for (int k = 0; k < NV; ++k) st[MCG_S_E + k] += t[k];
       if (therm_end) st[MCG_S_EREF] = st[MCG_S_E];
@@ -1679,11 +1689,12 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
     // Lane g of the first wave stands for group g (unsplit replicas) or for
     // the replica (part 0 of a split one) in the end-of-sweep work.  M' of a
     // row (for <M M'>) passes from frame to frame through slot f % 3 of
-    // a.mprev3: frame f of a row is taken after the barrier of its sweep,
-    // reads slot (f-1) % 3 and writes slot f % 3; the writer of frame f-1
-    // stored before it arrived at this barrier, and a workgroup is never more
-    // than one sweep ahead, so three slots never collide.  A[MPREV] seeds the
-    // slots of the rows held at the start and takes the last frame's M'.
+    // a.mprev3: frame f of a row is taken after the barrier of its sweep (or
+    // before it, see frames_after), reads slot (f-1) % 3 and writes slot
+    // f % 3; the writer of frame f-1 stored before it arrived at a barrier
+    // the reader has passed, and a workgroup is never more than one sweep
+    // ahead, so three slots never collide.  A[MPREV] seeds the slots of the
+    // rows held at the start and takes the last frame's M'.
     const int lane = threadIdx.x & 63;
     const int nfr = pi.np == 1 ? R : pi.part == 0 ? 1 : 0;  // groups whose frames this workgroup takes
     const bool l_grp = threadIdx.x < 64 && lane < nfr;
@@ -1713,6 +1724,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
       const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
       const bool pt_now = a.pt_every > 0 && ((k + 1) % a.pt_every) == 0;
       const int rowg_l = row0_l + s_rung[rep_l];
+      const int64_t fcur = (k - a.nthermal + 1) / a.spf - 1 + a.frame0;
       if (pl.rem > 0) {  // the pool's parameters of this sweep (rows move with tempering)
         if (lt == 0) s_par[g] = make_float4(bg, hg, __uint_as_float(key1), 0.0f);
         __syncthreads();
@@ -1753,13 +1765,27 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
         for (int i = lt; i < nex; i += TPR)
           export_site<NC>(xo + (size_t)i * NC, lraw, n, ex[i]);
       }
-      const int64_t fcur = (k - a.nthermal + 1) / a.spf - 1 + a.frame0;
       if (meas && l_grp && !frames_after) {
         take_frame<NC>(G, B, a, s_st, s_frm, lane, fcur, rowg_l, nrow, frame_mprev3<NC>(a, fcur, rowg_l, nrow));
         last_row = rowg_l;
       }
+      // the swap words of the first ladder are drawn while the barrier
+      // completes (they do not depend on the energies)
+      float pt_u0 = 0.0f;
       if (a.has_split || pt_now) {
-        if (!grid_barrier(a.bar, (uint32_t)(a.bar_base + nbar + 1), B.err, s_flag)) return;
+        const uint32_t epoch = (uint32_t)(a.bar_base + nbar + 1);
+        grid_arrive(a.bar, epoch);
+        if (pt_now && nlad > 0) {
+          const int64_t swap = a.philox_base + k;
+          const int i = (int)threadIdx.x - pt0;
+          const int t = (int)(swap & 1) + 2 * i;
+          if (i >= 0 && t + 1 < s_lad[2]) {
+            const u32x4 x = philox4x32_10(u32x4{(uint32_t)t, (uint32_t)swap, (uint32_t)((uint64_t)swap >> 32), TAG_PT},
+                                          B.seed_lo, B.seed_hi ^ (0x80000000u | (uint32_t)(s_lad[0] & 0x7FFFFFFF)));
+            pt_u0 = u24(x.x);
+          }
+        }
+        if (!grid_wait(a.bar, epoch, B.err, s_flag)) return;
         ++nbar;
       }
       // M' of the frame's row, read first so that it overlaps the import
@@ -1835,10 +1861,14 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
           const double d = (ba - bb) * (Ea - Eb);
           bool acc = d >= 0.0;
           if (!acc) {
-            const u32x4 x =
-                philox4x32_10(u32x4{(uint32_t)t, (uint32_t)swap, (uint32_t)((uint64_t)swap >> 32), TAG_PT},
-                              B.seed_lo, B.seed_hi ^ (0x80000000u | (uint32_t)L));
-            acc = u24(x.x) < exp_neg((float)d);
+            float u = pt_u0;  // drawn during the barrier for the first ladder
+            if (li > 0 || i >= (int)blockDim.x - pt0) {
+              const u32x4 x =
+                  philox4x32_10(u32x4{(uint32_t)t, (uint32_t)swap, (uint32_t)((uint64_t)swap >> 32), TAG_PT},
+                                B.seed_lo, B.seed_hi ^ (0x80000000u | (uint32_t)L));
+              u = u24(x.x);
+            }
+            acc = u < exp_neg((float)d);
           }
           if (acc) {
             s_who[r0 + t] = rb;
@@ -2894,10 +2924,14 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
   if (X.all_reps.upload(X.hall.data(), X.hall.size() * 4, s)) return -1;
   // ladders
   if (P->pt_every > 0) {
-    std::vector<int32_t> seen(B->nrow, 0);
+    std::vector<int32_t> seen(B->nrow, 0), lgraph(B->nrow, -1);
     for (int r = 0; r < B->nrep; ++r) {
       const mcg_replica& R = B->reps_host[r];
-      if (R.ladder < 0 || R.nrung < 2 || seen[R.row0]) continue;
+      if (R.ladder < 0 || R.nrung < 2) continue;
+      if (lgraph[R.row0] >= 0 && lgraph[R.row0] != R.graph)
+        return fail("replica %d: a tempering ladder holds replicas of one lattice", r);
+      lgraph[R.row0] = R.graph;
+      if (seen[R.row0]) continue;
       seen[R.row0] = 1;
       X.hl_row0.push_back(R.row0);
       X.hl_n.push_back(R.nrung);
