@@ -2580,6 +2580,10 @@ int plan_lds(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool force_stream
   if (work == 0) return 0;
   const int ncu = std::max(1, X.ncu);
   std::vector<char> nosplit(B->ngraph, 0);
+  // automatic splits only where the launch can be persistent: otherwise a
+  // launch with split replicas runs one sweep and restages them every sweep,
+  // which costs more than the imbalance it removes
+  const bool auto_split = !(B->meas & MEAS_SPIN_OBS) && B->recompute_every <= 0 && !P->no_persist;
   struct Split {
     int gi, rep, np;
   };
@@ -2602,7 +2606,7 @@ int plan_lds(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool force_stream
         int np = 1;
         if (P->parts >= 2)
           np = P->parts;
-        else if (P->parts == 0 && !nosplit[gi] && G.n >= 8192 && G.n > t)
+        else if (P->parts == 0 && auto_split && !nosplit[gi] && G.n >= 8192 && G.n > t)
           np = std::min({MAX_PARTS, (int)std::ceil(G.n / t), G.n / 4096});
         if (np >= 2) {
           gnp[gi] = np;
