@@ -107,6 +107,7 @@ struct SIO<int8_t> {
 // one replica's spins; PT is ST* (LDS, inferred) or GAS ST* (HBM)
 template <typename ST, typename PT = ST*>
 struct Spins {
+  typedef ST value_type;
   PT p;
   int n;
   __device__ __forceinline__ float get(int i, int c) const { return SIO<ST>::ld(p + (size_t)c * n + i); }
@@ -334,7 +335,7 @@ __device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, in
         dE = fmaf(Dz, q2, dE);
       }
     }
-    dE = fmaf(-h, NC == 3 ? d2 : d0, dE);
+    if (h != 0.0f) dE = fmaf(-h, NC == 3 ? d2 : d0, dE);  // h = 0 would add an exact 0
     const float db = dE * beta;
     if (db <= 0.0f || u24(w_c) < exp_neg(-db)) {
       float nn = n0 * n0;
@@ -348,9 +349,16 @@ __device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, in
       sp.set(i, 1, n1);
       if constexpr (NC == 3) sp.set(i, 2, n2);
       dEacc += dE;
-      dMacc[0] += sp.get(i, 0) - s0;
-      dMacc[1] += sp.get(i, 1) - s1;
-      if constexpr (NC == 3) dMacc[2] += sp.get(i, 2) - s2;
+      if constexpr (std::is_same<typename SV::value_type, float>::value) {
+        // FP32 storage keeps the value as computed
+        dMacc[0] += n0 - s0;
+        dMacc[1] += n1 - s1;
+        if constexpr (NC == 3) dMacc[2] += n2 - s2;
+      } else {
+        dMacc[0] += sp.get(i, 0) - s0;
+        dMacc[1] += sp.get(i, 1) - s1;
+        if constexpr (NC == 3) dMacc[2] += sp.get(i, 2) - s2;
+      }
     }
   }
 }
