@@ -2,11 +2,16 @@
 // Heisenberg spins on arbitrary bond graphs.  See include/mcgpu.h for the ABI.
 //
 // Kernels
-//   k_lds<NC,ST>      persistent sweep: each workgroup owns R replicas of one
-//                     lattice, stages their spins in LDS once, runs many
+//   k_lds<NC,ST,MODE,FULL,PERSIST>
+//                     LDS sweep: each workgroup owns R replicas of one
+//                     lattice (or one part of a replica split over 2-4
+//                     workgroups), stages their spins in LDS once, runs many
 //                     graph-coloured Metropolis sweeps with in-kernel fused
-//                     measurement, writes spins back once.  HBM traffic per
-//                     sweep ~0; bound by VALU/LDS.
+//                     measurement, writes spins back once.  PERSIST: every
+//                     workgroup resident, a grid barrier per sweep, tempering
+//                     and part exchange inside the kernel.  HBM traffic per
+//                     sweep ~1 MB of bookkeeping; bound by VALU issue and the
+//                     per-sweep synchronisation.
 //   k_pass<NC,ST>     streaming colour pass over all replicas (spins in HBM /
 //                     L2), for lattices too large for LDS; also the "plain"
 //                     one-launch-per-replica-per-colour baseline.
