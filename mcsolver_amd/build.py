@@ -47,16 +47,18 @@ def build_lib(force: bool = True, verbose: bool = False) -> str:
     return LIB
 
 
-def build_oracle(with_reference: bool = True) -> None:
+def build_oracle(with_reference: bool = True, force: bool = True) -> None:
     """make -C oracle (our restatement + checker); with_reference also builds
-    oracle/_ref from /root/reference when that tree exists here."""
+    oracle/_ref from /root/reference when that tree exists here. force
+    recompiles even when the outputs are newer than their sources."""
     od = os.path.join(ROOT, "oracle")
-    r = subprocess.run(["make", "-s", "-C", od], capture_output=True, text=True)
+    mk = ["make", "-s"] + (["-B"] if force else []) + ["-C", od]
+    r = subprocess.run(mk, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("oracle build failed")
     if with_reference and os.path.isdir("/root/reference/mcsolver"):
-        r = subprocess.run(["make", "-s", "-C", od, "ref"], capture_output=True, text=True)
+        r = subprocess.run(mk + ["ref"], capture_output=True, text=True)
         if r.returncode != 0:
             sys.stderr.write(r.stdout + r.stderr)
             raise RuntimeError("reference build (oracle/_ref) failed")
