@@ -98,6 +98,20 @@ def cpu_baseline(target_s: float = 10.0):
             "per_core": tot / sum(r[1] for r in res)}
 
 
+def tuned_options(a, b, pt_every):
+    """Plan options for Batch.run: the command line's, or with --autotune the
+    fastest of the candidate plans timed on this batch (the batch is left as
+    it was)."""
+    if not a.autotune:
+        return dict(parts=a.parts, no_persist=False, reps_per_block=0)
+    from mcsolver_amd.autotune import autotune
+    r = autotune(b, 1, pt_every, trial_sweeps=64)
+    for t in r.trials:
+        print("autotune %s: %s" % (t.options, "%.4f ms/sweep" % t.ms_per_sweep if t.ms_per_sweep is not None
+                                    else "rejected (%s)" % t.error), file=sys.stderr, flush=True)
+    return r.best
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +129,9 @@ def main():
     ap.add_argument("--store", default="f32", choices=["f32", "f16"])
     ap.add_argument("--parts", type=int, default=0,
                     help="workgroups per LDS replica: 0 = balance automatically, 1 = never split")
+    ap.add_argument("--autotune", action="store_true",
+                    help="choose the launch plan by timing candidate plans on the batch first (outside the timed "
+                         "region; results do not depend on the plan)")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -144,14 +161,15 @@ def main():
     wl = W.build("xy_ladder_scan", nframes=K + Wm + KT, nbins=16)
     b = wl.batch
     b.init()
-    b.run(0, Wm, 1, pt_every=wl.pt_every, block=a.block, parts=a.parts)
+    opts = tuned_options(a, b, wl.pt_every)
+    b.run(0, Wm, 1, pt_every=wl.pt_every, block=a.block, **opts)
     torch.cuda.synchronize()
 
     def plan(nsw, sweep0, timing=None):
         p = L.Plan()
         p.algorithm, p.nthermal, p.nsweep, p.sweeps_per_frame, p.pt_every = L.METROPOLIS, 0, nsw, 1, wl.pt_every
         p.kernel, p.block, p.sweep0, p.frame0 = L.KERNEL_AUTO, a.block, sweep0, sweep0
-        p.parts = a.parts
+        p.parts, p.no_persist, p.reps_per_block = opts["parts"], int(opts["no_persist"]), opts["reps_per_block"]
         if timing is not None:
             p.timing = C.cast(timing, C.POINTER(C.c_double))
         return p
@@ -192,6 +210,8 @@ def main():
                       "rungs_per_ladder": 64, "replicas": len(b.reps), "spins_per_sweep": ups, "pt_every": 1,
                       "measure_every": 1, "spin_storage": "fp32", "accumulators": "fp64"},
            "roofline": roof, "cpu_baseline": cpu}
+    if a.autotune:
+        out["plan"] = opts
     if not a.no_plain:
         wp = W.build("xy_ladder_scan", nframes=a.plain_steps, nbins=1)
         wp.batch.init()
@@ -222,14 +242,15 @@ def other_workload(a):
     wl = W.build(a.workload, store=a.store, nframes=K + Wm + KT, nbins=16, meas=meas)
     b = wl.batch
     b.init()
-    b.run(0, Wm, 1, parts=a.parts)
+    opts = tuned_options(a, b, 0)
+    b.run(0, Wm, 1, block=a.block, **opts)
     torch.cuda.synchronize()
 
     def plan(nsw, sweep0, timing=None):
         p = L.Plan()
         p.algorithm, p.nthermal, p.nsweep, p.sweeps_per_frame = L.METROPOLIS, 0, nsw, 1
         p.kernel, p.block, p.sweep0, p.frame0 = L.KERNEL_AUTO, a.block, sweep0, sweep0
-        p.parts = a.parts
+        p.parts, p.no_persist, p.reps_per_block = opts["parts"], int(opts["no_persist"]), opts["reps_per_block"]
         if timing is not None:
             p.timing = C.cast(timing, C.POINTER(C.c_double))
         return p
@@ -268,6 +289,8 @@ def other_workload(a):
                         "avg_launch_us": 1e3 * k_ms / max(k_launch, 1), "launches": int(k_launch),
                         "algorithmic_bytes_per_launch": bpu * k_upd / max(k_launch, 1)},
            "cpu_baseline": None}
+    if a.autotune:
+        out["plan"] = opts
     print(json.dumps(out), flush=True)
 
 

@@ -1,0 +1,115 @@
+"""Measured choice of the launch plan of a batch.
+
+mcMain.py runs one process per temperature (win.py:56-156 hands (T, H) points
+to a multiprocessing Pool) and has nothing to tune.  Here one call runs every
+replica, and the planner of libmcgpu deals replicas to workgroups by a fixed
+rule (sites per CU, automatic splits of large replicas, persistent launches
+where every workgroup is resident).  autotune() replaces that rule by a
+measurement: it times a few trial sweeps of each candidate plan on the batch
+itself and returns the fastest.
+
+The choice does not change the trajectory.  Philox counters are keyed by
+replica, site group, sweep and colour, never by workgroup or thread, so every
+plan gives the same spins (the GPU tests compare plans bit for bit).  The
+tracked energy and magnetisation are FP64 sums of per-thread FP32 partial
+sums, and which sites share a thread depends on the plan, so frame
+observables agree to FP32 rounding (about 1e-7 relative) and a tempering swap
+could differ only where its acceptance ratio lies within that rounding of its
+uniform draw.  The trials do advance the batch, so every device buffer a run
+writes is saved first and restored after each trial.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import torch
+
+from . import _lib as L
+from .engine import Batch
+
+# (parts, no_persist, reps_per_block): automatic splits or none or two parts,
+# persistent launches on or off, the planner's packing or one replica per
+# workgroup
+DEFAULT_CANDIDATES: Sequence[Dict] = tuple(
+    dict(parts=p, no_persist=np_, reps_per_block=rb)
+    for p in (0, 1, 2) for np_ in (False, True) for rb in (0, 1))
+
+
+@dataclass
+class Trial:
+    options: Dict
+    ms_per_sweep: Optional[float]      # None: the plan was rejected
+    error: str = ""
+
+
+@dataclass
+class TuneResult:
+    best: Dict
+    trials: List[Trial] = field(default_factory=list)
+
+
+def _mutable(b: Batch) -> List[torch.Tensor]:
+    """Every device buffer that mcg_run writes."""
+    t = [b.spins, b.state, b.acc, b.bins, b.gacc, b.pt_acc, b.err, b.rung, b.who]
+    return t + [x for x in (b.wstamp, b.wlist) if x is not None]
+
+
+def autotune(b: Batch, sweeps_per_frame: int = 1, pt_every: int = 0, trial_sweeps: int = 64,
+             candidates: Optional[Sequence[Dict]] = None) -> TuneResult:
+    """Time each candidate's Batch.run options on trial_sweeps measured sweeps
+    (frames every sweeps_per_frame sweeps, tempering every pt_every sweeps, as
+    the production run will have them) and return the fastest.  Each candidate
+    runs once untimed, then once timed with HIP events on the current stream.
+    A candidate the planner rejects, or whose run raises the device error flag,
+    is skipped.  The batch's spins, tracked state, accumulators, bins, ladders
+    and error flag are as before the call when it returns."""
+    if trial_sweeps < 1 or sweeps_per_frame < 1:
+        raise ValueError("trial_sweeps and sweeps_per_frame must be >= 1")
+    cands = list(DEFAULT_CANDIDATES if candidates is None else candidates)
+    if not cands:
+        raise ValueError("no candidate plans")
+    nframe = max(1, trial_sweeps // sweeps_per_frame)
+    bufs = _mutable(b)
+    saved = [x.clone() for x in bufs]
+    nframes0 = b.b.nframes
+    b.set_nframes(nframe)
+    s = torch.cuda.current_stream()
+    out = TuneResult(best={})
+    seen = set()
+    try:
+        for opt in cands:
+            key = tuple(sorted(opt.items()))
+            if key in seen:
+                continue
+            seen.add(key)
+            try:
+                times = []
+                for _ in range(2):
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(s)
+                    b.run(0, nframe, sweeps_per_frame, pt_every=pt_every, **opt)
+                    e1.record(s)
+                    e1.synchronize()
+                    times.append(e0.elapsed_time(e1))
+                    flag = b.error_flag()
+                    for x, y in zip(bufs, saved):
+                        x.copy_(y)
+                    if flag:
+                        raise RuntimeError("device error flag 0x%x" % flag)
+                out.trials.append(Trial(dict(opt), times[-1] / (nframe * sweeps_per_frame)))
+            except RuntimeError as e:
+                for x, y in zip(bufs, saved):
+                    x.copy_(y)
+                out.trials.append(Trial(dict(opt), None, str(e)))
+    finally:
+        for x, y in zip(bufs, saved):
+            x.copy_(y)
+        b.set_nframes(nframes0)
+        torch.cuda.synchronize()
+    ok = [t for t in out.trials if t.ms_per_sweep is not None]
+    if not ok:
+        raise RuntimeError("autotune: every candidate plan failed (%s)" %
+                           "; ".join(t.error for t in out.trials))
+    out.best = dict(min(ok, key=lambda t: t.ms_per_sweep).options)
+    return out
