@@ -26,6 +26,11 @@ python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-plain]
 heis_triangular_dmi128: triangular DMI 128^2 x 1024 (T, B); heis_cubic64:
 cubic 64^3 x 128 T) and prints the same fields for it (without the plain and
 CPU legs); the default is the XY ladder scan above.
+
+--workload cubic_wolff_vs_metropolis compares Wolff cluster updates with
+batched Metropolis on the cubic 64^3 Heisenberg ferromagnet (16 independent
+replicas at --tau-T each): frames/s, tau_int of |M| and E from per-frame
+series, and independent samples/s = replicas x frames/s / (2 tau_int).
 """
 from __future__ import annotations
 
@@ -125,7 +130,13 @@ def main():
                     help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--workload", default="xy_ladder_scan",
                     choices=["xy_ladder_scan", "ising_square32", "heis_honeycomb64", "heis_triangular_dmi128",
-                             "heis_cubic64"])
+                             "heis_cubic64", "cubic_wolff_vs_metropolis"])
+    ap.add_argument("--tau-T", type=float, default=1.44,
+                    help="cubic_wolff_vs_metropolis: temperature (T_c of the 3D Heisenberg ferromagnet is ~1.443)")
+    ap.add_argument("--tau-seconds", type=float, default=40.0,
+                    help="cubic_wolff_vs_metropolis: GPU time per method for the measured series")
+    ap.add_argument("--wolff-clusters", type=int, default=32,
+                    help="cubic_wolff_vs_metropolis: Wolff clusters per replica between frames")
     ap.add_argument("--store", default="f32", choices=["f32", "f16"])
     ap.add_argument("--parts", type=int, default=0,
                     help="workgroups per LDS replica: 0 = balance automatically, 1 = never split")
@@ -139,6 +150,9 @@ def main():
               "so N>1 is the same kernel on 1/N of the batch; multi-GPU runs are not part of this build.",
               file=sys.stderr)
         sys.exit(2)
+    if a.workload == "cubic_wolff_vs_metropolis":
+        wolff_vs_metropolis(a)
+        return
     if a.workload != "xy_ladder_scan":
         other_workload(a)
         return
@@ -291,6 +305,89 @@ def other_workload(a):
            "cpu_baseline": None}
     if a.autotune:
         out["plan"] = opts
+    print(json.dumps(out), flush=True)
+
+
+def _series(wl, algorithm, ncl, pre_metropolis, nthermal, chunk, fmax, budget_s, label):
+    """Thermalise (pre_metropolis Metropolis sweeps, then nthermal steps of the
+    method), then measure one frame per step in chunks until fmax frames or
+    budget_s seconds of GPU time.  Returns (frames, ms, bins[nrow, frames, q],
+    spins flipped per replica during the measured steps)."""
+    import torch
+    from mcsolver_amd import _lib as L
+    b = wl.batch
+    b.init()
+    if pre_metropolis:
+        b.run(pre_metropolis, 0, 1, algorithm=L.METROPOLIS)
+    b.run(nthermal, 0, 1, algorithm=algorithm, clusters_per_sweep=ncl, sweep0=pre_metropolis)
+    torch.cuda.synchronize()
+    flips0 = b.state_data()[:, L.S_ACC].copy()
+    s0 = pre_metropolis + nthermal
+    done, ms = 0, 0.0
+    while done < fmax and ms < budget_s * 1e3:
+        n = min(chunk, fmax - done)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.run(0, n, 1, algorithm=algorithm, clusters_per_sweep=ncl, sweep0=s0 + done, frame0=done)
+        e1.record()
+        e1.synchronize()
+        ms += e0.elapsed_time(e1)
+        done += n
+        print("%s: %d frames, %.1f s" % (label, done, ms / 1e3), file=sys.stderr, flush=True)
+    assert b.error_flag() == 0
+    return done, ms, b.bin_data()[:, :done, :], b.state_data()[:, L.S_ACC] - flips0
+
+
+def wolff_vs_metropolis(a):
+    """The cubic 64^3 Heisenberg comparison: Wolff single-cluster updates (one
+    workgroup per replica) against batched Metropolis, 16 independent replicas
+    each at one temperature, by tau_int-corrected throughput.  For each
+    replica tau_int of |M| and of E is estimated from its per-frame series;
+    independent samples/s = replicas x frames/s / (2 tau_int)."""
+    import numpy as np
+    import torch
+    from mcsolver_amd import _lib as L
+    from mcsolver_amd import workloads as W
+    from mcsolver_amd.analysis import tau_int
+
+    torch.cuda.set_device(0)
+    nrep, T = 16, a.tau_T
+    out = {"metric": "independent samples/s (tau_int-corrected), Wolff vs batched Metropolis",
+           "config": {"workload": "Heisenberg cubic 64^3, J=-1, random start", "T": T, "replicas": nrep,
+                      "wolff_clusters_per_frame": a.wolff_clusters, "gpu_seconds_per_method": a.tau_seconds},
+           "data": "synthetic"}
+    # Wolff starts from a Metropolis-equilibrated state: from a random start
+    # its clusters stay small and it orders slowly
+    runs = (("metropolis", L.METROPOLIS, 0, 0, 10_000, 5000, 400_000),
+            ("wolff", L.WOLFF, a.wolff_clusters, 10_000, 50, 25, 40_000))
+    for name, alg, ncl, pre, nth, chunk, fmax in runs:
+        wl = W.cubic_replicas(T, nrep, wolff=(alg == L.WOLFF), nframes=fmax)
+        n_sites = wl.batch.graphs[0].n
+        frames, ms, bins, flips = _series(wl, alg, ncl, pre, nth, chunk, fmax, a.tau_seconds, name)
+        fps = frames / (ms / 1e3)
+        # sweep-equivalents per frame: one sweep (Metropolis), spins flipped / N (Wolff)
+        sweq = 1.0 if alg == L.METROPOLIS else float(flips.mean()) / (n_sites * frames)
+        res = {"frames": frames, "seconds": ms / 1e3, "frames_per_s": fps, "sweep_equivalents_per_frame": sweq}
+        if alg == L.WOLFF:
+            res["mean_flipped_cluster"] = float(flips.mean()) / (frames * ncl)
+        for q, obs in ((2, "M"), (0, "E")):
+            taus = [tau_int(bins[r, :, q]) for r in range(nrep)]
+            tv = np.array([t for t, _ in taus])
+            closed = sum(1 for _, w in taus if w < frames - 1)
+            res["tau_" + obs] = float(tv.mean())
+            res["tau_%s_sweeps" % obs] = float(tv.mean()) * sweq
+            res["tau_%s_sem" % obs] = float(tv.std(ddof=1) / np.sqrt(nrep))
+            res["windows_closed_" + obs] = closed
+            res["frames_over_tau_" + obs] = frames / float(tv.mean())
+            res["samples_per_s_" + obs] = nrep * fps / (2.0 * float(tv.mean()))
+        res["mean_abs_M"] = float(bins[:, :, 2].mean())
+        if alg == L.METROPOLIS:
+            res["spin_updates_per_s"] = fps * wl.updates_per_sweep
+        out[name] = res
+        del wl
+        torch.cuda.empty_cache()
+    for obs in ("M", "E"):
+        out["wolff_over_metropolis_" + obs] = out["wolff"]["samples_per_s_" + obs] / out["metropolis"]["samples_per_s_" + obs]
     print(json.dumps(out), flush=True)
 
 

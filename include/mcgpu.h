@@ -50,7 +50,8 @@ enum {
   MCG_A_SIJR = 27, MCG_A_ER, MCG_A_ER2, MCG_A_COUNT, MCG_A_LAST, MCG_A_TOT, MCG_A_ABSM,
   MCG_NACC = 34
 };
-/* tracked per-replica state (FP64) */
+/* tracked per-replica state (FP64); Wolff adds the sizes of accepted
+ * clusters (spins flipped) to MCG_S_ACC and of all grown clusters to MCG_S_TRY */
 enum { MCG_S_E = 0, MCG_S_M = 1 /* 3 */, MCG_S_EREF = 4, MCG_S_DRIFT = 5, MCG_S_ACC = 6, MCG_S_TRY = 7,
        MCG_NSTATE = 8 };
 /* per-bin accumulators: e, e^2, M, M^2, M^4 */

@@ -19,7 +19,7 @@ MEAS_PAIRS, MEAS_Q, MEAS_RNORM, MEAS_GROUPS, ISING_RELE = 1, 2, 4, 8, 16
 A_SI, A_SJ, A_SIJ, A_ZI, A_ZJ, A_ZT, A_HI, A_HJ, A_HT, A_Q = 7, 10, 13, 14, 15, 16, 17, 18, 19, 20
 A_SIR, A_SJR, A_SIJR, A_ER, A_ER2, A_COUNT, A_LAST, A_TOT, A_ABSM = 21, 24, 27, 28, 29, 30, 31, 32, 33
 NACC = 34
-S_E, S_M, S_EREF, S_DRIFT = 0, 1, 4, 5
+S_E, S_M, S_EREF, S_DRIFT, S_ACC, S_TRY = 0, 1, 4, 5, 6, 7
 NSTATE = 8
 NBINQ = 5
 KERNEL_AUTO, KERNEL_LDS, KERNEL_STREAM, KERNEL_PLAIN = 0, 1, 2, 3

@@ -51,3 +51,30 @@ def batch_estimates(batch, temps, n_sites_per_row) -> list:
         d["U4"], d["U4_err"] = jackknife(means, lambda m: m[3] ** 2 / m[4])
         out.append(d)
     return out
+
+
+def tau_int(x, c: float = 5.0):
+    """Integrated autocorrelation time of a series, in units of its spacing:
+    tau = 1/2 + sum_{t=1..W} rho(t), with rho the normalised autocovariance
+    (divided by n, computed by FFT) and the window W chosen self-consistently
+    as the smallest W >= c * tau(W) (Madras and Sokal, J. Stat. Phys. 50, 109,
+    1988).  Returns (tau, W); W = n - 1 means the window never closed and the
+    series is too short for tau.  A run of n frames holds about n / (2 tau)
+    independent samples."""
+    x = np.asarray(x, np.float64)
+    n = x.size
+    if n < 2:
+        return 0.5, 0
+    x = x - x.mean()
+    var = float(x @ x) / n
+    if var == 0.0:
+        return 0.5, 0
+    m = 1 << (2 * n - 1).bit_length()
+    f = np.fft.rfft(x, m)
+    rho = np.fft.irfft(f * np.conj(f), m)[:n] / (n * var)
+    tau = 0.5
+    for w in range(1, n):
+        tau += float(rho[w])
+        if w >= c * tau:
+            return tau, w
+    return tau, n - 1

@@ -2120,7 +2120,8 @@ __global__ void k_pt(mcg_batch B, const int32_t* __restrict__ ladder_row0, const
 }
 
 // ---------------------------------------------------------------------------
-// k_wolff: one block per replica, ncl clusters, frontier BFS in global memory
+// k_wolff: one 1024-thread block per replica, ncl clusters, frontier BFS in
+// global memory with one work item per (frontier site, link)
 // (heisenbergLib.c:310-439, xyLib.c:256-380, isingLib.c:165-236)
 // ---------------------------------------------------------------------------
 template <int NC, typename ST>
@@ -2174,12 +2175,18 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
     while (true) {
       const int lo = s_lo, hi = s_hi;
       if (lo >= hi) break;
-      for (int f = lo + tid; f < hi; f += blockDim.x) {
-        const int o = __hip_atomic_load(mem + f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // one work item per (frontier site, link): every item is one short
+      // dependent chain (stamp test, bond draw, stamp exchange)
+      const uint32_t zl = (uint32_t)G.zmax;
+      const uint32_t nitem = (uint32_t)(hi - lo) * zl;
+      for (uint32_t it = tid; it < nitem; it += blockDim.x) {
+        const uint32_t fq = it / zl;
+        const int l = (int)(it - fq * zl);
+        const int o = __hip_atomic_load(mem + lo + (int)fq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         float so[3];
         spin_vec<NC>(sp, o, G.cls, G.clsdat, so);
         const float sdo = -(so[0] * r0 + so[1] * r1 + so[2] * r2);
-        for (int l = 0; l < G.zmax; ++l) {
+        {
           const uint32_t wd = G.link[(size_t)o * G.zmax + l];
           const int t = (int)(wd & 0xFFFFFFu);
           if (t == o) continue;
@@ -2220,9 +2227,15 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
     }
     const int cnt = s_cnt;
     // cluster acceptance energy (field / anisotropy correction)
+    // O(n): one work item per (cluster site, link) plus one per site for the
+    // single-ion and field terms
     double d = 0.0;
-    for (int f = tid; f < cnt; f += blockDim.x) {
-      const int b = __hip_atomic_load(mem + f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t zi = NC == 1 ? 1u : (uint32_t)G.zmax + 1u;
+    const uint32_t nacc = (uint32_t)cnt * zi;
+    for (uint32_t it = tid; it < nacc; it += blockDim.x) {
+      const uint32_t fq = it / zi;
+      const int l = (int)(it - fq * zi);
+      const int b = __hip_atomic_load(mem + (int)fq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       float sb[3];
       spin_vec<NC>(sp, b, G.cls, G.clsdat, sb);
       if constexpr (NC == 1) {
@@ -2230,7 +2243,7 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
       } else {
         const float sdb = -(sb[0] * r0 + sb[1] * r1 + sb[2] * r2);
         const float pb[3] = {sb[0] + sdb * r0, sb[1] + sdb * r1, sb[2] + sdb * r2};
-        for (int l = 0; l < G.zmax; ++l) {
+        if (l < G.zmax) {
           const uint32_t wd = G.link[(size_t)b * G.zmax + l];
           const int t = (int)(wd & 0xFFFFFFu);
           const uint32_t cw = G.lcid ? (uint32_t)G.lcid[(size_t)b * G.zmax + l] : (wd >> 24);
@@ -2251,18 +2264,19 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
                                  pb[2] * (J[6] * r0 + J[7] * r1 + J[8] * r2)));
           else
             d += (double)src;
+        } else {
+          // single-ion term with the half step s + sdn*r (heisenbergLib.c:418)
+          const int ci = G.cls[b];
+          const float* Dd = G.clsdat + ci * 8 + 1;
+          const float tr[3] = {sdb * r0, sdb * r1, sdb * r2};
+          float o = 0.0f;
+          for (int k = 0; k < NC; ++k) {
+            const float q = sb[k] + tr[k];
+            o += Dd[k] * (q * q - sb[k] * sb[k]);
+          }
+          o -= h * tr[NC == 3 ? 2 : 0];
+          d += (double)o;
         }
-        // single-ion term with the half step s + sdn*r (heisenbergLib.c:418)
-        const int ci = G.cls[b];
-        const float* Dd = G.clsdat + ci * 8 + 1;
-        const float tr[3] = {sdb * r0, sdb * r1, sdb * r2};
-        float o = 0.0f;
-        for (int k = 0; k < NC; ++k) {
-          const float q = sb[k] + tr[k];
-          o += Dd[k] * (q * q - sb[k] * sb[k]);
-        }
-        o -= h * tr[NC == 3 ? 2 : 0];
-        d += (double)o;
       }
     }
     for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o, 64);
@@ -2274,6 +2288,10 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
       const float db = (float)(t * (double)beta);
       const float u = (float)s_acc * 0x1p-24f;
       s_acc = (db <= 0.0f || u < exp_neg(-db)) ? 1 : 0;
+      // cluster statistics: sites flipped and sites grown
+      double* st = B.state + (size_t)rep * MCG_NSTATE;
+      if (s_acc) st[MCG_S_ACC] += (double)cnt;
+      st[MCG_S_TRY] += (double)cnt;
     }
     __syncthreads();
     if (s_acc) {
@@ -2963,13 +2981,17 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
 
   if (P->algorithm == MCG_WOLFF) {
     if (!B->wstamp || !B->wlist) return fail("Wolff needs wstamp/wlist scratch");
-    int64_t nmax = 0;
-    for (int gi = 0; gi < B->ngraph; ++gi) nmax = std::max<int64_t>(nmax, B->graphs_host[gi].n);
+    int64_t nmax = 0, wmax = 0;
+    for (int gi = 0; gi < B->ngraph; ++gi) {
+      nmax = std::max<int64_t>(nmax, B->graphs_host[gi].n);
+      wmax = std::max<int64_t>(wmax, (int64_t)B->graphs_host[gi].n * (B->graphs_host[gi].zmax + 1));
+    }
+    if (wmax >= ((int64_t)1 << 32)) return fail("Wolff: sites x (links + 1) must stay below 2^32");
     for (int64_t k = 0; k < total; ++k) {
       const int64_t kk = P->sweep0 + k;
       // clusters per sweep: plan value or n of the replica's graph (use max n)
       const int ncl = P->clusters_per_sweep > 0 ? P->clusters_per_sweep : (int)nmax;
-      hipLaunchKernelGGL((k_wolff<NC, ST>), dim3((unsigned)nall), dim3(256), 0, s, *B,
+      hipLaunchKernelGGL((k_wolff<NC, ST>), dim3((unsigned)nall), dim3(1024), 0, s, *B,
                          (const int32_t*)X.all_reps.p, kk * (int64_t)ncl, ncl);
       HIPCHK(hipGetLastError());
       const bool meas = k >= nth && ((k - nth + 1) % spf) == 0;

@@ -159,6 +159,23 @@ def build(name: str, store: str = "f32", nbins: int = 32, nframes: int = 1, meas
                     "i8" if nc == 1 else store, pt_every, desc)
 
 
+def cubic_replicas(T: float, nrep: int = 16, wolff: bool = False, nframes: int = 1, store: str = "f32",
+                   device="cuda") -> Workload:
+    """nrep independent replicas of the Heisenberg cubic 64^3 ferromagnet
+    (J = -1, random start) at one temperature, each on its own row with one
+    bin per frame, so that the bins hold every replica's per-frame series of
+    e, e^2, |M|, M^2, M^4 (for autocorrelation times)."""
+    rng = np.random.default_rng(SEED + 1)
+    Lt = lat.cubic(64, J=-1.0)
+    reps = [ReplicaSpec(0, k) for k in range(nrep)]
+    inits = [_init(3, Lt.n, Lt.S, "random", rng) for _ in range(nrep)]
+    temps = np.full(nrep, float(T))
+    b = Batch(3, [_graph(3, Lt)], reps, 1.0 / np.maximum(temps, 0.1), [0.0] * nrep, inits, store=store,
+              nbins=nframes, nframes=nframes, seed=SEED, wolff=wolff, device=device)
+    return Workload("heis_cubic64", 3, b, [Lt], temps, np.zeros(nrep), np.zeros(nrep, int), store, 0,
+                    "Heisenberg cubic 64^3, J=-1, random start, %d replicas at T=%g" % (nrep, T))
+
+
 # algorithmic HBM bytes per site update: each site's state is read once and
 # written once per sweep (couplings are per-lattice tables, Philox is
 # counter-based, measurement is fused)
