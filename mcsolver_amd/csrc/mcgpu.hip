@@ -2141,6 +2141,21 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
   __shared__ double s_red[16];
   __shared__ int s_acc;
   const int tid = threadIdx.x;
+  // isotropic exchange (every tensor c*I on the spin components) and no
+  // single-ion term: with h = 0 the cluster's acceptance energy
+  // (heisenbergLib.c:403-423) is zero in exact arithmetic, so the cluster is
+  // accepted without evaluating it (Ising: zero whenever h = 0)
+  bool iso_ok = true;
+  if constexpr (NC > 1) {
+    for (int c = tid; c < G.ncoup; c += blockDim.x) {
+      const float* J = G.coup + c * 9;
+      iso_ok = iso_ok && J[1] == 0.0f && J[3] == 0.0f && J[0] == J[4];
+      if (NC == 3) iso_ok = iso_ok && J[2] == 0.0f && J[5] == 0.0f && J[6] == 0.0f && J[7] == 0.0f && J[8] == J[0];
+    }
+    for (int c = tid; c < G.ncls; c += blockDim.x)
+      for (int k = 0; k < NC; ++k) iso_ok = iso_ok && G.clsdat[c * 8 + 1 + k] == 0.0f;
+  }
+  const bool skip_acc = __syncthreads_and(iso_ok ? 1 : 0) != 0 && h == 0.0f;
 
   for (int c = 0; c < ncl; ++c) {
     const uint64_t cl = (uint64_t)(cl0 + c);
@@ -2231,7 +2246,7 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
     // single-ion and field terms
     double d = 0.0;
     const uint32_t zi = NC == 1 ? 1u : (uint32_t)G.zmax + 1u;
-    const uint32_t nacc = (uint32_t)cnt * zi;
+    const uint32_t nacc = skip_acc ? 0u : (uint32_t)cnt * zi;
     for (uint32_t it = tid; it < nacc; it += blockDim.x) {
       const uint32_t fq = it / zi;
       const int l = (int)(it - fq * zi);

@@ -417,6 +417,19 @@ int mcref_cb_wolff(int nc, int store, long n, int zmax, const int *nbr, const do
                    uint32_t serial, uint32_t seed_lo, uint32_t seed_hi, long c0, long ncl, float *sp, long *sizes) {
   cb_graph g;
   cb_build(&g, nc, store, n, zmax, nbr, J, S, D, diag_only);
+  /* isotropic exchange and no single-ion term: with h = 0 the acceptance
+     energy is zero in exact arithmetic and is not evaluated (as the engine) */
+  int iso = 1;
+  for (long b = 0; b < n && nc > 1; b++) {
+    for (int l = 0; l < zmax; l++) {
+      if (nbr[b * zmax + l] < 0) continue;
+      const float *Jm = g.cj + (b * zmax + l) * 9;
+      if (Jm[1] != 0.0f || Jm[3] != 0.0f || Jm[0] != Jm[4]) iso = 0;
+      if (nc == 3 && (Jm[2] != 0.0f || Jm[5] != 0.0f || Jm[6] != 0.0f || Jm[7] != 0.0f || Jm[8] != Jm[0])) iso = 0;
+    }
+    for (int k = 0; k < nc; k++)
+      if (g.D[b * 3 + k] != 0.0f) iso = 0;
+  }
   char *in = malloc((size_t)n);
   long *q = malloc(sizeof(long) * n);
   /* the engine indexes sites by their position in colour order: the seed is
@@ -485,7 +498,7 @@ int mcref_cb_wolff(int nc, int store, long n, int zmax, const int *nbr, const do
     }
     if (sizes) sizes[c] = tail;
     double d = 0.0;
-    for (long m = 0; m < tail; m++) {
+    for (long m = 0; m < (iso && h == 0.0f ? 0 : tail); m++) {
       long b = q[m];
       if (nc == 1) {
         d += (double)(2.0f * h * (sp[b] * g.S[b]));
