@@ -1,7 +1,7 @@
 """ORACLE — test infrastructure only.  ctypes wrapper of oracle/libmcref.so.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
-this module, and only as the checker / CPU baseline.
+this module, and only to verify results or to time the CPU baseline.
 """
 from __future__ import annotations
 

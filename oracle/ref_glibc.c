@@ -1,7 +1,7 @@
 /*
  * ORACLE — test infrastructure only.  Nothing in the product links, loads or
  * calls this file; only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg use it, and only as the checker / CPU baseline.
+ * cpu_baseline leg use it, and only to verify results or to time the CPU baseline.
  *
  * FP64 CPU restatement of the reference Monte Carlo engines
  *   mcsolver/isingLib.c       (Ising,      MCMainFunction  isingLib.c:259-451)
