@@ -2044,12 +2044,12 @@ __global__ __launch_bounds__(256) void k_pass(mcg_batch B, const int32_t* __rest
 // 4 measure frame, 8 reset drift
 // ---------------------------------------------------------------------------
 template <int NC, typename ST>
-__global__ __launch_bounds__(256) void k_measure(mcg_batch B, const int32_t* __restrict__ rep_list, int mode,
-                                                 int64_t frame) {
+__global__ __launch_bounds__(1024) void k_measure(mcg_batch B, const int32_t* __restrict__ rep_list, int mode,
+                                                  int64_t frame) {
   const int rep = rep_list[blockIdx.x];
   const mcg_replica rp = B.reps[rep];
   const mcg_graph G = B.graphs[rp.graph];
-  __shared__ double s_red[4 * RK];
+  __shared__ double s_red[16 * RK];
   __shared__ double s_frm[FRM];
   __shared__ double st[MCG_NSTATE];
   const Spins<ST> sp{(ST*)B.spins + rp.spin_off, G.n};
@@ -2871,7 +2871,7 @@ int launch_pass(const mcg_batch* B, const int32_t* dreps, int nreps, int maxn, i
 template <int NC, typename ST>
 int launch_measure(const mcg_batch* B, const int32_t* dreps, int nreps, int mode, int64_t frame, hipStream_t s) {
   if (nreps == 0) return 0;
-  hipLaunchKernelGGL((k_measure<NC, ST>), dim3((unsigned)nreps), dim3(256), 0, s, *B, dreps, mode, frame);
+  hipLaunchKernelGGL((k_measure<NC, ST>), dim3((unsigned)nreps), dim3(1024), 0, s, *B, dreps, mode, frame);
   HIPCHK(hipGetLastError());
   return 0;
 }
