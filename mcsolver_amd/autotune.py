@@ -73,6 +73,7 @@ def autotune(b: Batch, sweeps_per_frame: int = 1, pt_every: int = 0, trial_sweep
     bufs = _mutable(b)
     saved = [x.clone() for x in bufs]
     nframes0 = b.b.nframes
+    sweep0, cl_end0 = b._next_sweep, b._wolff_cl_end
     b.set_nframes(nframe)
     s = torch.cuda.current_stream()
     out = TuneResult(best={})
@@ -88,7 +89,7 @@ def autotune(b: Batch, sweeps_per_frame: int = 1, pt_every: int = 0, trial_sweep
                 for _ in range(2):
                     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                     e0.record(s)
-                    b.run(0, nframe, sweeps_per_frame, pt_every=pt_every, **opt)
+                    b.run(0, nframe, sweeps_per_frame, pt_every=pt_every, sweep0=sweep0, **opt)
                     e1.record(s)
                     e1.synchronize()
                     times.append(e0.elapsed_time(e1))
@@ -106,6 +107,7 @@ def autotune(b: Batch, sweeps_per_frame: int = 1, pt_every: int = 0, trial_sweep
         for x, y in zip(bufs, saved):
             x.copy_(y)
         b.set_nframes(nframes0)
+        b._next_sweep, b._wolff_cl_end = sweep0, cl_end0
         torch.cuda.synchronize()
     ok = [t for t in out.trials if t.ms_per_sweep is not None]
     if not ok:

@@ -33,6 +33,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -2394,20 +2395,49 @@ int check_batch(const mcg_batch* B) {
   return 0;
 }
 
-// launch-time resources kept for one call
+// A prepared run: the launch plan (host), and the device scratch the
+// launches use beyond the batch's own arrays, carved from one region in
+// 256-byte aligned pieces:
+//   tables  work items, their replicas, replica lists, ladders, split and
+//           workgroup-ladder tables; written once when the run is prepared
+//   state   arrival counters, barrier words, published energies and M',
+//           exported sites; every run starts them from zero (one fill kernel)
+// Preparing plans and uploads; running only launches kernels, so repeated
+// runs of one batch pay no planning, allocation or host copy.
 struct RunCtx {
-  DevBuf items, item_reps, all_reps, stream_reps, lad_row0, lad_n, sync, xpart, halo;
+  // plan (host)
   std::vector<WorkItem> hitems;
   std::vector<int32_t> hitem_reps, hstream, hall, hhalo;
+  std::vector<int32_t> hl_row0, hl_n;
   int block = 0;
   int ncu = 0;
   int nslots = 0;  // split replicas (arrival counters)
   int max_export = 0;  // longest export list of a part
   bool persist = false;  // persistent launches (barrier + tempering inside)
-  DevBuf bar, eslot, mslot, xbuf, rep_np, blk_lad, mprev3;
   int mode_begin[5] = {0, 0, 0, 0, 0};  // work items of link mode m: [mode_begin[m], mode_begin[m+1])
   size_t lds_max = 0;
-  std::vector<int32_t> hl_row0, hl_n;
+  double lds_updates_per_sweep = 0;
+  int device = 0;
+  // what the plan was made for: a run whose plan or batch differs in any of
+  // these is refused
+  int32_t k_kernel = 0, k_block = 0, k_rpb = 0, k_parts = 0, k_nopersist = 0, k_pt = 0;
+  int32_t k_nc = 0, k_store = 0, k_ngraph = 0, k_nrep = 0, k_nrow = 0, k_meas = 0, k_rec = 0;
+  const void *k_graphs = nullptr, *k_reps = nullptr, *k_graphs_host = nullptr, *k_reps_host = nullptr;
+  // device region: [0, tab_bytes) tables, [reset_off, reset_off + reset_bytes) state
+  unsigned char* dev = nullptr;
+  size_t dev_bytes = 0, tab_bytes = 0, reset_off = 0, reset_bytes = 0;
+  std::vector<unsigned char> hstage;  // the tables as uploaded
+  // offsets of the pieces, then their device addresses
+  size_t o_items = 0, o_item_reps = 0, o_stream = 0, o_all = 0, o_lrow0 = 0, o_ln = 0, o_halo = 0, o_repnp = 0,
+         o_blklad = 0, o_sync = 0, o_xpart = 0, o_bar = 0, o_eslot = 0, o_mslot = 0, o_xbuf = 0, o_mprev3 = 0;
+  const WorkItem* items = nullptr;
+  const ItemRep* item_reps = nullptr;
+  const int32_t *stream_reps = nullptr, *all_reps = nullptr, *lad_row0 = nullptr, *lad_n = nullptr,
+                *halo = nullptr, *blk_lad = nullptr;
+  const uint8_t* rep_np = nullptr;
+  uint32_t *sync = nullptr, *bar = nullptr;
+  double *xpart = nullptr, *eslot = nullptr, *mslot = nullptr, *mprev3 = nullptr;
+  void* xbuf = nullptr;
 };
 
 // HIP-event bracketing of the sweep-kernel launches (plan->timing)
@@ -2771,8 +2801,7 @@ int launch_lds_range(const mcg_batch* B, RunCtx& X, const SweepArgs& a, hipStrea
   if (PERSIST) bytes += persist_lds_bytes(B->nrow, B->nrep);
   set_lds_attr<NC, ST, MODE, FULL, PERSIST>(bytes);
   hipExtLaunchKernelGGL((k_lds<NC, ST, MODE, FULL, PERSIST>), dim3((unsigned)(i1 - i0)), dim3(X.block),
-                        (uint32_t)bytes, s, e0, e1, 0, *B, (const WorkItem*)X.items.p + i0,
-                        (const ItemRep*)X.item_reps.p, a);
+                        (uint32_t)bytes, s, e0, e1, 0, *B, X.items + i0, X.item_reps, a);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -2898,7 +2927,7 @@ int stream_sweeps(const mcg_batch* B, const mcg_plan* P, RunCtx& X, int64_t k0, 
   if (X.hstream.empty()) return 0;
   int ncolmax = 0;
   const int maxn = max_colour_count(B, X.hstream, &ncolmax);
-  const int32_t* dreps = (const int32_t*)X.stream_reps.p;
+  const int32_t* dreps = X.stream_reps;
   const int nrs = (int)X.hstream.size();
   for (int64_t q = 0; q < nsw; ++q) {
     const int64_t k = k0 + q;
@@ -2937,43 +2966,29 @@ int stream_sweeps(const mcg_batch* B, const mcg_plan* P, RunCtx& X, int64_t k0, 
   return 0;
 }
 
+// piece of the device region, 256-byte aligned
+size_t take(size_t* off, size_t bytes) {
+  const size_t o = *off;
+  *off += (bytes + 255) & ~(size_t)255;
+  return o;
+}
+
+// Plan the run of B under P and lay out its device region: fills everything
+// in X but the device addresses (bind_region).  X.dev_bytes is the size of
+// the region the run needs.
 template <int NC, typename ST>
-int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
-  RunCtx X;
+int prepare_typed(const mcg_batch* B, const mcg_plan* P, RunCtx& X) {
+  HIPCHK(hipGetDevice(&X.device));
   if (device_lds_max(&X.lds_max)) return -1;
-  {
-    int dev = 0;
-    HIPCHK(hipGetDevice(&dev));
-    HIPCHK(hipDeviceGetAttribute(&X.ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  }
+  HIPCHK(hipDeviceGetAttribute(&X.ncu, hipDeviceAttributeMultiprocessorCount, X.device));
   const bool force_stream = P->kernel == 2 || P->kernel == 3;
-  const bool plain = P->kernel == 3;
   if (plan_lds(B, P, X, force_stream)) return -1;
-  if (X.nslots > 0) {
-    const int64_t total_sweeps = (int64_t)P->nthermal + (int64_t)P->nsweep * std::max(1, P->sweeps_per_frame);
-    if (total_sweeps * MAX_PARTS >= ((int64_t)1 << 32))
-      return fail("too many sweeps in one call for split replicas (%lld); split the call", (long long)total_sweeps);
-    std::vector<uint32_t> zero((size_t)X.nslots * SYNC_STRIDE, 0u);
-    if (X.sync.upload(zero.data(), zero.size() * 4, s)) return -1;
-    std::vector<double> zx((size_t)X.nslots * MAX_PARTS * 4, 0.0);
-    if (X.xpart.upload(zx.data(), zx.size() * 8, s)) return -1;
-    if (X.halo.upload(X.hhalo.data(), X.hhalo.size() * 4, s)) return -1;
-  }
   if (P->kernel == 1 && !X.hstream.empty())
     return fail("LDS-resident kernel requested but %zu replica(s) do not fit in LDS", X.hstream.size());
-  if (X.items.upload(X.hitems.data(), X.hitems.size() * sizeof(WorkItem), s)) return -1;
+  // tempering ladders (row0, rungs), each of one lattice
+  X.hl_row0.clear();
+  X.hl_n.clear();
   {
-    std::vector<ItemRep> recs(X.hitem_reps.size());
-    for (size_t i = 0; i < recs.size(); ++i) {
-      const mcg_replica& R = B->reps_host[X.hitem_reps[i]];
-      recs[i] = ItemRep{R.spin_off, X.hitem_reps[i], (int32_t)R.serial, R.row0, 0};
-    }
-    if (X.item_reps.upload(recs.data(), recs.size() * sizeof(ItemRep), s)) return -1;
-  }
-  if (X.stream_reps.upload(X.hstream.data(), X.hstream.size() * 4, s)) return -1;
-  if (X.all_reps.upload(X.hall.data(), X.hall.size() * 4, s)) return -1;
-  // ladders
-  if (P->pt_every > 0) {
     std::vector<int32_t> seen(B->nrow, 0), lgraph(B->nrow, -1);
     for (int r = 0; r < B->nrep; ++r) {
       const mcg_replica& R = B->reps_host[r];
@@ -2986,76 +3001,26 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
       X.hl_row0.push_back(R.row0);
       X.hl_n.push_back(R.nrung);
     }
-    if (X.lad_row0.upload(X.hl_row0.data(), X.hl_row0.size() * 4, s)) return -1;
-    if (X.lad_n.upload(X.hl_n.data(), X.hl_n.size() * 4, s)) return -1;
   }
-  const int spf = P->sweeps_per_frame > 0 ? P->sweeps_per_frame : 1;
-  const int64_t nth = P->nthermal;
-  const int64_t total = nth + (int64_t)P->nsweep * spf;
-  const int nall = B->nrep;
-
-  if (P->algorithm == MCG_WOLFF) {
-    if (!B->wstamp || !B->wlist) return fail("Wolff needs wstamp/wlist scratch");
-    int64_t nmax = 0, wmax = 0;
-    for (int gi = 0; gi < B->ngraph; ++gi) {
-      nmax = std::max<int64_t>(nmax, B->graphs_host[gi].n);
-      wmax = std::max<int64_t>(wmax, (int64_t)B->graphs_host[gi].n * (B->graphs_host[gi].zmax + 1));
-    }
-    if (wmax >= ((int64_t)1 << 32)) return fail("Wolff: sites x (links + 1) must stay below 2^32");
-    for (int64_t k = 0; k < total; ++k) {
-      const int64_t kk = P->sweep0 + k;
-      // clusters per sweep: plan value or n of the replica's graph (use max n)
-      const int ncl = P->clusters_per_sweep > 0 ? P->clusters_per_sweep : (int)nmax;
-      hipLaunchKernelGGL((k_wolff<NC, ST>), dim3((unsigned)nall), dim3(1024), 0, s, *B,
-                         (const int32_t*)X.all_reps.p, kk * (int64_t)ncl, ncl);
-      HIPCHK(hipGetLastError());
-      const bool meas = k >= nth && ((k - nth + 1) % spf) == 0;
-      const bool therm_end = (k + 1 == nth);
-      const int mode = 1 | 8 | (therm_end ? 2 : 0) | (meas ? 4 : 0);
-      if (meas || therm_end || k + 1 == total) {
-        const int64_t frame = meas ? (k - nth + 1) / spf - 1 + P->frame0 : -1;
-        if (launch_measure<NC, ST>(B, (const int32_t*)X.all_reps.p, nall, mode, frame, s)) return -1;
-      }
-      if (P->pt_every > 0 && ((k + 1) % P->pt_every) == 0 && !X.hl_row0.empty()) {
-        if (launch_measure<NC, ST>(B, (const int32_t*)X.all_reps.p, nall, 1 | 8, -1, s)) return -1;
-        hipLaunchKernelGGL(k_pt, dim3((unsigned)X.hl_row0.size()), dim3(64), 0, s, *B,
-                           (const int32_t*)X.lad_row0.p, (const int32_t*)X.lad_n.p, (int)X.hl_row0.size(),
-                           P->sweep0 + k);
-        HIPCHK(hipGetLastError());
-      }
-    }
-    return 0;
-  }
-
-  LaunchTimer T;
-  T.on = P->timing != nullptr;
-  double lds_updates_per_sweep = 0;
+  X.lds_updates_per_sweep = 0;
   for (const WorkItem& w : X.hitems)
-    if (w.part == 0) lds_updates_per_sweep += (double)w.nrep * B->graphs_host[w.graph].n;
+    if (w.part == 0) X.lds_updates_per_sweep += (double)w.nrep * B->graphs_host[w.graph].n;
   // persistent launches when possible (P->no_persist turns them off)
   X.persist = false;
-  if (!P->no_persist && !force_stream && check_persist<NC, ST>(B, P, X, &X.persist)) return -1;
+  if (P->algorithm == MCG_METROPOLIS && !P->no_persist && !force_stream && check_persist<NC, ST>(B, P, X, &X.persist))
+    return -1;
+  // workgroup ladder lists of a persistent launch: the ladders of each
+  // workgroup's replicas; the first workgroup of a ladder owns it
+  std::vector<int32_t> bl;
+  std::vector<uint8_t> np;
   if (X.persist) {
-    std::vector<uint32_t> z(BAR_WORDS, 0u);
-    if (X.bar.upload(z.data(), z.size() * 4, s)) return -1;
-    std::vector<double> ze((size_t)2 * B->nrep * MAX_PARTS * 4, 0.0);
-    if (X.eslot.upload(ze.data(), (size_t)2 * B->nrep * MAX_PARTS * 8, s)) return -1;
-    if (X.mslot.upload(ze.data(), (size_t)2 * B->nrep * MAX_PARTS * 3 * 8, s)) return -1;
-    const size_t xb = (size_t)2 * std::max(X.nslots, 1) * MAX_PARTS * std::max(X.max_export, 1) * NC * sizeof(ST);
-    std::vector<unsigned char> zx(xb, 0);
-    if (X.xbuf.upload(zx.data(), xb, s)) return -1;
-    std::vector<double> zm((size_t)3 * B->nrow, 0.0);
-    if (X.mprev3.upload(zm.data(), zm.size() * 8, s)) return -1;
-    std::vector<uint8_t> np(B->nrep, 1);
+    np.assign(B->nrep, 1);
     for (const WorkItem& w : X.hitems)
       if (w.nparts > 1) np[X.hitem_reps[w.off]] = w.nparts;
-    if (X.rep_np.upload(np.data(), np.size(), s)) return -1;
-    // ladders of each workgroup's replicas; the first workgroup of a ladder
-    // owns it
     std::vector<int32_t> lad_of_row(B->nrow, -1);
     for (size_t L = 0; L < X.hl_row0.size(); ++L) lad_of_row[X.hl_row0[L]] = (int32_t)L;
     const size_t nb = X.hitems.size();
-    std::vector<int32_t> bl(nb + 1, 0);
+    bl.assign(nb + 1, 0);
     std::vector<char> owned(X.hl_row0.size(), 0);
     for (size_t i = 0; i < nb; ++i) {
       const WorkItem& w = X.hitems[i];
@@ -3074,9 +3039,165 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
       bl[i + 1] = (int32_t)(bl.size() - (nb + 1));
     }
     for (size_t i = 0; i <= nb; ++i) bl[i] += (int32_t)(nb + 1);
-    if (X.blk_lad.upload(bl.data(), bl.size() * 4, s)) return -1;
+  }
+  std::vector<ItemRep> recs(X.hitem_reps.size());
+  for (size_t i = 0; i < recs.size(); ++i) {
+    const mcg_replica& R = B->reps_host[X.hitem_reps[i]];
+    recs[i] = ItemRep{R.spin_off, X.hitem_reps[i], (int32_t)R.serial, R.row0, 0};
+  }
+  // layout: tables first, then the state every run clears
+  size_t off = 0;
+  X.o_items = take(&off, X.hitems.size() * sizeof(WorkItem));
+  X.o_item_reps = take(&off, recs.size() * sizeof(ItemRep));
+  X.o_stream = take(&off, X.hstream.size() * 4);
+  X.o_all = take(&off, X.hall.size() * 4);
+  X.o_lrow0 = take(&off, X.hl_row0.size() * 4);
+  X.o_ln = take(&off, X.hl_n.size() * 4);
+  X.o_halo = take(&off, X.hhalo.size() * 4);
+  X.o_repnp = take(&off, np.size());
+  X.o_blklad = take(&off, bl.size() * 4);
+  X.tab_bytes = off;
+  X.reset_off = off;
+  X.o_sync = take(&off, (size_t)X.nslots * SYNC_STRIDE * 4);
+  X.o_xpart = take(&off, (size_t)X.nslots * MAX_PARTS * 4 * 8);
+  if (X.persist) {
+    X.o_bar = take(&off, (size_t)BAR_WORDS * 4);
+    X.o_eslot = take(&off, (size_t)2 * B->nrep * MAX_PARTS * 8);
+    X.o_mslot = take(&off, (size_t)2 * B->nrep * MAX_PARTS * 3 * 8);
+    X.o_xbuf = take(&off, (size_t)2 * std::max(X.nslots, 1) * MAX_PARTS * std::max(X.max_export, 1) * NC * sizeof(ST));
+    X.o_mprev3 = take(&off, (size_t)3 * B->nrow * 8);
+  }
+  X.reset_bytes = off - X.reset_off;
+  X.dev_bytes = off;
+  X.hstage.assign(X.tab_bytes, 0);
+  auto put = [&](size_t o, const void* src, size_t n) {
+    if (n) std::memcpy(X.hstage.data() + o, src, n);
+  };
+  put(X.o_items, X.hitems.data(), X.hitems.size() * sizeof(WorkItem));
+  put(X.o_item_reps, recs.data(), recs.size() * sizeof(ItemRep));
+  put(X.o_stream, X.hstream.data(), X.hstream.size() * 4);
+  put(X.o_all, X.hall.data(), X.hall.size() * 4);
+  put(X.o_lrow0, X.hl_row0.data(), X.hl_row0.size() * 4);
+  put(X.o_ln, X.hl_n.data(), X.hl_n.size() * 4);
+  put(X.o_halo, X.hhalo.data(), X.hhalo.size() * 4);
+  put(X.o_repnp, np.data(), np.size());
+  put(X.o_blklad, bl.data(), bl.size() * 4);
+  // the key
+  X.k_kernel = P->kernel;
+  X.k_block = P->block;
+  X.k_rpb = P->reps_per_block;
+  X.k_parts = P->parts;
+  X.k_nopersist = P->no_persist;
+  X.k_pt = P->pt_every > 0;
+  X.k_nc = B->nc;
+  X.k_store = B->store;
+  X.k_ngraph = B->ngraph;
+  X.k_nrep = B->nrep;
+  X.k_nrow = B->nrow;
+  X.k_meas = B->meas;
+  X.k_rec = B->recompute_every;
+  X.k_graphs = B->graphs;
+  X.k_reps = B->reps;
+  X.k_graphs_host = B->graphs_host;
+  X.k_reps_host = B->reps_host;
+  return 0;
+}
+
+// Give X its device region (X.dev_bytes at dev) and copy the tables there,
+// on stream s.  The caller keeps X.hstage alive until the copy is done.
+int bind_region(RunCtx& X, unsigned char* dev, hipStream_t s) {
+  X.dev = dev;
+  auto at = [&](size_t o) { return (void*)(dev + o); };
+  X.items = (const WorkItem*)at(X.o_items);
+  X.item_reps = (const ItemRep*)at(X.o_item_reps);
+  X.stream_reps = (const int32_t*)at(X.o_stream);
+  X.all_reps = (const int32_t*)at(X.o_all);
+  X.lad_row0 = (const int32_t*)at(X.o_lrow0);
+  X.lad_n = (const int32_t*)at(X.o_ln);
+  X.halo = (const int32_t*)at(X.o_halo);
+  X.rep_np = (const uint8_t*)at(X.o_repnp);
+  X.blk_lad = (const int32_t*)at(X.o_blklad);
+  X.sync = (uint32_t*)at(X.o_sync);
+  X.xpart = (double*)at(X.o_xpart);
+  X.bar = (uint32_t*)at(X.o_bar);
+  X.eslot = (double*)at(X.o_eslot);
+  X.mslot = (double*)at(X.o_mslot);
+  X.xbuf = at(X.o_xbuf);
+  X.mprev3 = (double*)at(X.o_mprev3);
+  if (X.tab_bytes && hipMemcpyAsync(dev, X.hstage.data(), X.tab_bytes, hipMemcpyHostToDevice, s) != hipSuccess)
+    return fail("hipMemcpyAsync of the run tables failed");
+  return 0;
+}
+
+// Is X the plan of this batch and plan?
+int check_key(const mcg_batch* B, const mcg_plan* P, const RunCtx& X) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev != X.device) return fail("run prepared on device %d, called on device %d", X.device, dev);
+  if (P->kernel != X.k_kernel || P->block != X.k_block || P->reps_per_block != X.k_rpb || P->parts != X.k_parts ||
+      P->no_persist != X.k_nopersist || (P->pt_every > 0) != (X.k_pt != 0))
+    return fail("plan differs from the prepared one in kernel, block, reps_per_block, parts, no_persist or "
+                "tempering on/off; prepare a new run for it");
+  if (B->nc != X.k_nc || B->store != X.k_store || B->ngraph != X.k_ngraph || B->nrep != X.k_nrep ||
+      B->nrow != X.k_nrow || B->meas != X.k_meas || B->recompute_every != X.k_rec || B->graphs != X.k_graphs ||
+      B->reps != X.k_reps || B->graphs_host != X.k_graphs_host || B->reps_host != X.k_reps_host)
+    return fail("batch differs from the one the run was prepared for");
+  return 0;
+}
+
+// The launches of one run of a prepared context (no allocation, no host
+// copy): clear the run state, then the sweep / cluster / tempering launches.
+template <int NC, typename ST>
+int exec_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s, RunCtx& X) {
+  const bool plain = P->kernel == 3;
+  const int spf = P->sweeps_per_frame > 0 ? P->sweeps_per_frame : 1;
+  const int64_t nth = P->nthermal;
+  const int64_t total = nth + (int64_t)P->nsweep * spf;
+  const int nall = B->nrep;
+  if (X.nslots > 0 && total * MAX_PARTS >= ((int64_t)1 << 32))
+    return fail("too many sweeps in one call for split replicas (%lld); split the call", (long long)total);
+  if (X.reset_bytes) {
+    const size_t nw = X.reset_bytes / 4;
+    const unsigned g = (unsigned)std::min<size_t>((nw + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(256), 0, s, (uint32_t*)(X.dev + X.reset_off), nw, 0u);
+    HIPCHK(hipGetLastError());
   }
   const bool pt_on = P->pt_every > 0 && !X.hl_row0.empty();
+
+  if (P->algorithm == MCG_WOLFF) {
+    if (!B->wstamp || !B->wlist) return fail("Wolff needs wstamp/wlist scratch");
+    int64_t nmax = 0, wmax = 0;
+    for (int gi = 0; gi < B->ngraph; ++gi) {
+      nmax = std::max<int64_t>(nmax, B->graphs_host[gi].n);
+      wmax = std::max<int64_t>(wmax, (int64_t)B->graphs_host[gi].n * (B->graphs_host[gi].zmax + 1));
+    }
+    if (wmax >= ((int64_t)1 << 32)) return fail("Wolff: sites x (links + 1) must stay below 2^32");
+    for (int64_t k = 0; k < total; ++k) {
+      const int64_t kk = P->sweep0 + k;
+      // clusters per sweep: plan value or n of the replica's graph (use max n)
+      const int ncl = P->clusters_per_sweep > 0 ? P->clusters_per_sweep : (int)nmax;
+      hipLaunchKernelGGL((k_wolff<NC, ST>), dim3((unsigned)nall), dim3(1024), 0, s, *B, X.all_reps,
+                         kk * (int64_t)ncl, ncl);
+      HIPCHK(hipGetLastError());
+      const bool meas = k >= nth && ((k - nth + 1) % spf) == 0;
+      const bool therm_end = (k + 1 == nth);
+      const int mode = 1 | 8 | (therm_end ? 2 : 0) | (meas ? 4 : 0);
+      if (meas || therm_end || k + 1 == total) {
+        const int64_t frame = meas ? (k - nth + 1) / spf - 1 + P->frame0 : -1;
+        if (launch_measure<NC, ST>(B, X.all_reps, nall, mode, frame, s)) return -1;
+      }
+      if (pt_on && ((k + 1) % P->pt_every) == 0) {
+        if (launch_measure<NC, ST>(B, X.all_reps, nall, 1 | 8, -1, s)) return -1;
+        hipLaunchKernelGGL(k_pt, dim3((unsigned)X.hl_row0.size()), dim3(64), 0, s, *B, X.lad_row0, X.lad_n,
+                           (int)X.hl_row0.size(), P->sweep0 + k);
+        HIPCHK(hipGetLastError());
+      }
+    }
+    return 0;
+  }
+
+  LaunchTimer T;
+  T.on = P->timing != nullptr;
   // a persistent launch runs up to 4096 sweeps; otherwise split replicas
   // finish every launch (one sweep per launch) and tempering needs a launch
   // boundary every pt_every sweeps
@@ -3084,22 +3205,21 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
   int64_t bar_base = 0;
   for (int64_t k = 0; k < total;) {
     const int64_t ns = std::min<int64_t>(chunk, total - k);
-    SweepArgs a{k, P->sweep0, (int32_t)ns, (int32_t)nth, spf, P->frame0, (uint32_t*)X.sync.p, (double*)X.xpart.p,
-                (const int32_t*)X.halo.p};
+    SweepArgs a{k, P->sweep0, (int32_t)ns, (int32_t)nth, spf, P->frame0, X.sync, X.xpart, X.halo};
     if (X.persist) {
-      a.bar = (uint32_t*)X.bar.p;
+      a.bar = X.bar;
       a.bar_base = (int32_t)bar_base;
       a.pt_every = pt_on ? P->pt_every : 0;
       a.has_split = X.nslots > 0;
       a.nladder = (int32_t)X.hl_row0.size();
-      a.lad_row0 = (const int32_t*)X.lad_row0.p;
-      a.lad_n = (const int32_t*)X.lad_n.p;
-      a.blk_lad = (const int32_t*)X.blk_lad.p;
-      a.rep_np = (const uint8_t*)X.rep_np.p;
-      a.eslot = (double*)X.eslot.p;
-      a.mslot = (double*)X.mslot.p;
-      a.xbuf = X.xbuf.p;
-      a.mprev3 = (double*)X.mprev3.p;
+      a.lad_row0 = X.lad_row0;
+      a.lad_n = X.lad_n;
+      a.blk_lad = X.blk_lad;
+      a.rep_np = X.rep_np;
+      a.eslot = X.eslot;
+      a.mslot = X.mslot;
+      a.xbuf = X.xbuf;
+      a.mprev3 = X.mprev3;
       a.max_export = std::max(X.max_export, 1);
       a.nslots = std::max(X.nslots, 1);
       for (int64_t q = k; q < k + ns; ++q)
@@ -3110,18 +3230,44 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
     if (!X.hitems.empty()) {
       if (launch_lds<NC, ST>(B, X, a, s, T)) return -1;
       T.sweeps += (double)ns;
-      T.updates += lds_updates_per_sweep * (double)ns;
+      T.updates += X.lds_updates_per_sweep * (double)ns;
     }
     if (stream_sweeps<NC, ST>(B, P, X, k, ns, nth, spf, plain, s, T)) return -1;
     k += ns;
     if (!X.persist && pt_on && (k % P->pt_every) == 0) {
-      hipLaunchKernelGGL(k_pt, dim3((unsigned)X.hl_row0.size()), dim3(64), 0, s, *B,
-                         (const int32_t*)X.lad_row0.p, (const int32_t*)X.lad_n.p, (int)X.hl_row0.size(),
-                         P->sweep0 + k - 1);
+      hipLaunchKernelGGL(k_pt, dim3((unsigned)X.hl_row0.size()), dim3(64), 0, s, *B, X.lad_row0, X.lad_n,
+                         (int)X.hl_row0.size(), P->sweep0 + k - 1);
       HIPCHK(hipGetLastError());
     }
   }
   return T.finish(P->timing);
+}
+
+// mcg_run: prepare into a region of its own, run, wait, release
+template <int NC, typename ST>
+int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
+  RunCtx X;
+  if (prepare_typed<NC, ST>(B, P, X)) return -1;
+  void* dev = nullptr;
+  if (X.dev_bytes && hipMalloc(&dev, X.dev_bytes) != hipSuccess) return fail("hipMalloc(%zu) failed", X.dev_bytes);
+  int rc = bind_region(X, (unsigned char*)dev, s);
+  if (!rc) rc = exec_typed<NC, ST>(B, P, s, X);
+  // the region and the staged tables must outlive the launches
+  const hipError_t e = hipStreamSynchronize(s);
+  if (dev) (void)hipFree(dev);
+  if (!rc && e != hipSuccess) rc = fail("hipStreamSynchronize: %s", hipGetErrorString(e));
+  return rc;
+}
+
+template <int NC, typename ST>
+int prepare_runner(const mcg_batch* B, const mcg_plan* P, hipStream_t s, RunCtx& X) {
+  if (prepare_typed<NC, ST>(B, P, X)) return -1;
+  void* dev = nullptr;
+  if (X.dev_bytes && hipMalloc(&dev, X.dev_bytes) != hipSuccess) return fail("hipMalloc(%zu) failed", X.dev_bytes);
+  X.dev = (unsigned char*)dev;
+  if (bind_region(X, (unsigned char*)dev, s)) return -1;
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
 }
 
 template <int NC, typename ST>
@@ -3189,6 +3335,68 @@ extern "C" int mcg_run(const mcg_batch* B, const mcg_plan* P, void* stream) {
   if (P->algorithm != MCG_METROPOLIS && P->algorithm != MCG_WOLFF) return fail("unknown algorithm %d", P->algorithm);
   hipStream_t s = (hipStream_t)stream;
   DISPATCH(run_typed, B, P, s);
+}
+
+// A prepared run (mcg_runner_*): the RunCtx and the region it owns.
+struct mcg_runner {
+  RunCtx X;
+};
+
+namespace {
+template <int NC, typename ST>
+int runner_create_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s, mcg_runner* R) {
+  return prepare_runner<NC, ST>(B, P, s, R->X);
+}
+template <int NC, typename ST>
+int runner_run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s, mcg_runner* R) {
+  if (check_key(B, P, R->X)) return -1;
+  return exec_typed<NC, ST>(B, P, s, R->X);
+}
+void runner_free(mcg_runner* R) {
+  if (!R) return;
+  if (R->X.dev) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(R->X.device);
+    (void)hipDeviceSynchronize();  // no launch of this run may still read the region
+    (void)hipFree(R->X.dev);
+    (void)hipSetDevice(cur);
+  }
+  delete R;
+}
+}  // namespace
+
+extern "C" int mcg_runner_create(const mcg_batch* B, const mcg_plan* P, void* stream, mcg_runner** out) {
+  if (!out) return fail("null runner out-pointer");
+  *out = nullptr;
+  if (check_batch(B)) return -1;
+  if (!P) return fail("null plan");
+  if (P->algorithm != MCG_METROPOLIS && P->algorithm != MCG_WOLFF) return fail("unknown algorithm %d", P->algorithm);
+  mcg_runner* R = new (std::nothrow) mcg_runner();
+  if (!R) return fail("out of host memory");
+  hipStream_t s = (hipStream_t)stream;
+  const int rc = [&]() -> int { DISPATCH(runner_create_typed, B, P, s, R); }();
+  if (rc) {
+    runner_free(R);
+    return -1;
+  }
+  *out = R;
+  return 0;
+}
+
+extern "C" int mcg_runner_run(mcg_runner* R, const mcg_batch* B, const mcg_plan* P, void* stream) {
+  if (!R) return fail("null runner");
+  if (check_batch(B)) return -1;
+  if (!P) return fail("null plan");
+  if (P->nthermal < 0 || P->nsweep < 0) return fail("negative sweep counts");
+  if (P->algorithm != MCG_METROPOLIS && P->algorithm != MCG_WOLFF) return fail("unknown algorithm %d", P->algorithm);
+  hipStream_t s = (hipStream_t)stream;
+  DISPATCH(runner_run_typed, B, P, s, R);
+}
+
+extern "C" int mcg_runner_destroy(mcg_runner* R) {
+  runner_free(R);
+  return 0;
 }
 
 extern "C" int mcg_sweeps(const mcg_batch* B, const mcg_plan* P, int64_t sweep0, int32_t nsweeps, void* stream) {

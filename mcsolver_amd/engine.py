@@ -364,6 +364,22 @@ class Batch:
         b.reps_host = C.addressof(self.h_reps)
         self.b = b
         self.frames_per_row = max(1, nframes)
+        self.wolff = wolff
+        self._runners = {}        # plan shape -> prepared run (mcg_runner_*)
+        self._next_sweep = 0      # Philox sweep index after the last run
+        self._wolff_cl_end = 0    # first Wolff cluster id not yet used
+
+    def __del__(self):
+        self.close()
+
+    def close(self) -> None:
+        """Release the prepared runs (their device scratch)."""
+        rs, self._runners = getattr(self, "_runners", {}), {}
+        for h in rs.values():
+            try:
+                L.lib().mcg_runner_destroy(h)
+            except Exception:
+                pass
 
     # ---------------------------------------------------------------------
     @staticmethod
@@ -371,28 +387,79 @@ class Batch:
         return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
     def init(self) -> None:
+        """Tracked state from the current spins; clears accumulators, bins,
+        swap counts, the error flag and the Wolff stamps."""
         L.check(L.lib().mcg_init(C.byref(self.b), self._stream()), "mcg_init")
+        self._wolff_cl_end = 0
 
-    def run(self, nthermal: int, nsweep: int, sweeps_per_frame: int = 1, algorithm: int = L.METROPOLIS,
-            pt_every: int = 0, kernel: int = L.KERNEL_AUTO, block: int = 0, reps_per_block: int = 0,
-            clusters_per_sweep: int = 0, sweep0: int = 0, frame0: int = 0, parts: int = 0,
-            no_persist: bool = False) -> None:
-        """parts: workgroups per LDS replica (0 = balance automatically, 1 =
-        never split, 2..4 = split every LDS replica).  no_persist: never run
-        the LDS kernel as a persistent launch with tempering inside."""
+    @staticmethod
+    def plan(nthermal: int, nsweep: int, sweeps_per_frame: int = 1, algorithm: int = L.METROPOLIS,
+             pt_every: int = 0, kernel: int = L.KERNEL_AUTO, block: int = 0, reps_per_block: int = 0,
+             clusters_per_sweep: int = 0, sweep0: int = 0, frame0: int = 0, parts: int = 0,
+             no_persist: bool = False, timing=None) -> L.Plan:
         p = L.Plan()
         p.algorithm, p.nthermal, p.nsweep, p.sweeps_per_frame = algorithm, nthermal, nsweep, sweeps_per_frame
         p.pt_every, p.kernel, p.block, p.reps_per_block = pt_every, kernel, block, reps_per_block
         p.clusters_per_sweep, p.sweep0, p.frame0, p.parts = clusters_per_sweep, sweep0, frame0, parts
         p.no_persist = int(bool(no_persist))
-        L.check(L.lib().mcg_run(C.byref(self.b), C.byref(p), self._stream()), "mcg_run")
+        if timing is not None:
+            p.timing = C.cast(timing, C.POINTER(C.c_double))
+        return p
 
-    def sweeps(self, sweep0: int, nsweeps: int, algorithm: int = L.METROPOLIS, kernel: int = L.KERNEL_AUTO,
-               clusters_per_sweep: int = 0, block: int = 0, parts: int = 0) -> None:
-        p = L.Plan()
-        p.algorithm, p.kernel, p.clusters_per_sweep, p.sweeps_per_frame, p.block, p.parts = \
-            algorithm, kernel, clusters_per_sweep, 1, block, parts
-        L.check(L.lib().mcg_sweeps(C.byref(self.b), C.byref(p), sweep0, nsweeps, self._stream()), "mcg_sweeps")
+    def runner(self, p: L.Plan):
+        """The prepared run (mcg_runner_create) for this plan's shape: made on
+        first use, then reused, so that repeated runs only launch kernels."""
+        key = (p.kernel, p.block, p.reps_per_block, p.parts, p.no_persist, p.pt_every > 0, p.algorithm,
+               self.b.meas, self.b.recompute_every)
+        h = self._runners.get(key)
+        if h is None:
+            out = C.c_void_p()
+            L.check(L.lib().mcg_runner_create(C.byref(self.b), C.byref(p), self._stream(), C.byref(out)),
+                    "mcg_runner_create")
+            h = self._runners[key] = out
+        return h
+
+    def run_plan(self, p: L.Plan, prepared: bool = True) -> None:
+        """Run plan p: through its prepared run (default), or with mcg_run
+        (plans, allocates and waits every call)."""
+        ncl = p.clusters_per_sweep if p.clusters_per_sweep > 0 else max(g.n for g in self.graphs)
+        total = p.nthermal + p.nsweep * max(1, p.sweeps_per_frame)
+        if p.algorithm == L.WOLFF and self.wstamp is not None:
+            # cluster ids are (sweep index) x ncl + c; stamps of an earlier run
+            # with the same ids would be taken for members of the new cluster
+            if p.sweep0 * ncl < self._wolff_cl_end:
+                self.wstamp.zero_()
+            self._wolff_cl_end = (p.sweep0 + total) * ncl
+        if prepared:
+            L.check(L.lib().mcg_runner_run(self.runner(p), C.byref(self.b), C.byref(p), self._stream()),
+                    "mcg_runner_run")
+        else:
+            L.check(L.lib().mcg_run(C.byref(self.b), C.byref(p), self._stream()), "mcg_run")
+        self._next_sweep = p.sweep0 + total
+
+    def run(self, nthermal: int, nsweep: int, sweeps_per_frame: int = 1, algorithm: int = L.METROPOLIS,
+            pt_every: int = 0, kernel: int = L.KERNEL_AUTO, block: int = 0, reps_per_block: int = 0,
+            clusters_per_sweep: int = 0, sweep0: Optional[int] = None, frame0: int = 0, parts: int = 0,
+            no_persist: bool = False, prepared: bool = True) -> None:
+        """parts: workgroups per LDS replica (0 = balance automatically, 1 =
+        never split, 2..4 = split every LDS replica).  no_persist: never run
+        the LDS kernel as a persistent launch with tempering inside.  sweep0:
+        Philox sweep index of the first sweep; None continues after the last
+        run of this batch (0 for the first)."""
+        if sweep0 is None:
+            sweep0 = self._next_sweep
+        self.run_plan(self.plan(nthermal, nsweep, sweeps_per_frame, algorithm, pt_every, kernel, block,
+                                reps_per_block, clusters_per_sweep, sweep0, frame0, parts, no_persist),
+                      prepared=prepared)
+
+    def sweeps(self, sweep0: Optional[int], nsweeps: int, algorithm: int = L.METROPOLIS,
+               kernel: int = L.KERNEL_AUTO, clusters_per_sweep: int = 0, block: int = 0, parts: int = 0,
+               prepared: bool = True) -> None:
+        """nsweeps unmeasured sweeps (the same as mcg_sweeps)."""
+        if sweep0 is None:
+            sweep0 = self._next_sweep
+        self.run_plan(self.plan(nsweeps, 0, 1, algorithm, 0, kernel, block, 0, clusters_per_sweep, sweep0, 0,
+                                parts), prepared=prepared)
 
     def measure(self, frame: int) -> None:
         L.check(L.lib().mcg_measure(C.byref(self.b), frame, self._stream()), "mcg_measure")
