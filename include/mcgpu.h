@@ -2,7 +2,8 @@
  * libmcgpu — MI355X-native Monte Carlo sweep engine for classical spin models
  * (Ising, XY, Heisenberg).  Plain C ABI: POD structs, device pointers owned by
  * the caller, an explicit HIP stream, an int status (0 = ok), no exceptions
- * and no process-global mutable state across calls.
+ * and no process-global mutable state across calls (except the per-device
+ * scratch array of mcg_run_noalloc).
  *
  * Replaces the single entry point of the reference's three CPython
  * extensions,
@@ -212,6 +213,24 @@ typedef struct mcg_runner mcg_runner;
 int mcg_runner_create(const mcg_batch *host_batch, const mcg_plan *plan, void *stream, mcg_runner **out);
 int mcg_runner_run(mcg_runner *runner, const mcg_batch *host_batch, const mcg_plan *plan, void *stream);
 int mcg_runner_destroy(mcg_runner *runner);
+
+/*
+ * mcg_run without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same plan, launches and
+ * results as mcg_run; its device scratch (launch tables, arrival counters,
+ * barrier words, published energies) lives in a file-scope array of
+ * MCG_NOALLOC_SCRATCH_BYTES per device, which every device that loads the
+ * library holds until the process ends.  A run that needs more is refused
+ * with an error before anything is launched (configs of a few thousand
+ * replicas need well under 1 MiB).  Every call on a device shares that
+ * device's array and overwrites it: calls that overlap in time on one device
+ * (two streams, two threads) race; queue them on one stream.  It uploads
+ * the launch tables and waits for that copy (not for the launches), then
+ * returns when the launches are queued.  plan->timing must be NULL.
+ */
+#define MCG_NOALLOC_SCRATCH_BYTES (4u << 20)
+int mcg_run_noalloc(const mcg_batch *host_batch, const mcg_plan *plan, void *stream);
 
 /* Advance plan->nsweep sweeps without measuring (used between spin frames). */
 int mcg_sweeps(const mcg_batch *host_batch, const mcg_plan *plan, int64_t sweep0, int32_t nsweeps,

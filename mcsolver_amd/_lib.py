@@ -78,17 +78,19 @@ def lib() -> C.CDLL:
     L.mcg_runner_create.argtypes = [C.POINTER(Batch), C.POINTER(Plan), P, C.POINTER(C.c_void_p)]
     L.mcg_runner_run.argtypes = [C.c_void_p, C.POINTER(Batch), C.POINTER(Plan), P]
     L.mcg_runner_destroy.argtypes = [C.c_void_p]
+    L.mcg_run_noalloc.argtypes = [C.POINTER(Batch), C.POINTER(Plan), P]
     L.mcg_last_error.restype = C.c_char_p
     L.mcg_version.restype = C.c_int
     for f in (L.mcg_init, L.mcg_run, L.mcg_sweeps, L.mcg_measure, L.mcg_colour_graph, L.mcg_runner_create,
-              L.mcg_runner_run, L.mcg_runner_destroy):
+              L.mcg_runner_run, L.mcg_runner_destroy, L.mcg_run_noalloc):
         f.restype = C.c_int
     _lib = L
     return L
 
 
-EXPORTS = ("mcg_init", "mcg_run", "mcg_runner_create", "mcg_runner_run", "mcg_runner_destroy", "mcg_sweeps",
-           "mcg_measure", "mcg_colour_graph", "mcg_last_error", "mcg_version")
+EXPORTS = ("mcg_init", "mcg_run", "mcg_runner_create", "mcg_runner_run", "mcg_runner_destroy", "mcg_run_noalloc",
+           "mcg_sweeps", "mcg_measure", "mcg_colour_graph", "mcg_last_error", "mcg_version")
+NOALLOC_SCRATCH_BYTES = 4 << 20  # MCG_NOALLOC_SCRATCH_BYTES
 
 
 def check(rc: int, what: str) -> int:

@@ -33,6 +33,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -3259,6 +3260,55 @@ int run_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
   return rc;
 }
 
+// ---------------------------------------------------------------------------
+// mcg_run_noalloc: mcg_run's launches with their device scratch in a
+// file-scope array (allocated with the code object on every device that
+// loads the library, held until the process ends), for callers that cannot
+// allocate.  Host code never passes `scratch` itself to a kernel (in host
+// code the name stands for a host-side shadow): each device's address is
+// looked up once with hipGetSymbolAddress and kept in g_scratch_dev.
+// ---------------------------------------------------------------------------
+static constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)4 << 20;
+static_assert(NOALLOC_SCRATCH_BYTES == (size_t)MCG_NOALLOC_SCRATCH_BYTES, "header and library disagree");
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+static constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_dev[MAX_DEVICES];
+
+// the current device's copy of `scratch` (looked up on its first use)
+int scratch_region(unsigned char** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return fail("device %d outside 0..%d", dev, MAX_DEVICES - 1);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_dev[dev] = (unsigned char*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+template <int NC, typename ST>
+int run_noalloc_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s) {
+  if (P->timing) return fail("mcg_run_noalloc does not time launches: plan->timing must be NULL");
+  RunCtx X;
+  if (prepare_typed<NC, ST>(B, P, X)) return -1;
+  if (X.dev_bytes > NOALLOC_SCRATCH_BYTES)
+    return fail("this run needs %zu bytes of device scratch; mcg_run_noalloc has %zu (use mcg_run or a runner)",
+                X.dev_bytes, NOALLOC_SCRATCH_BYTES);
+  unsigned char* dev = nullptr;
+  if (scratch_region(&dev)) return -1;
+  if (bind_region(X, dev, s)) return -1;
+  // the tables are staged in X, which ends with this call: wait for their copy
+  HIPCHK(hipStreamSynchronize(s));
+  return exec_typed<NC, ST>(B, P, s, X);
+}
+
 template <int NC, typename ST>
 int prepare_runner(const mcg_batch* B, const mcg_plan* P, hipStream_t s, RunCtx& X) {
   if (prepare_typed<NC, ST>(B, P, X)) return -1;
@@ -3397,6 +3447,15 @@ extern "C" int mcg_runner_run(mcg_runner* R, const mcg_batch* B, const mcg_plan*
 extern "C" int mcg_runner_destroy(mcg_runner* R) {
   runner_free(R);
   return 0;
+}
+
+extern "C" int mcg_run_noalloc(const mcg_batch* B, const mcg_plan* P, void* stream) {
+  if (check_batch(B)) return -1;
+  if (!P) return fail("null plan");
+  if (P->nthermal < 0 || P->nsweep < 0) return fail("negative sweep counts");
+  if (P->algorithm != MCG_METROPOLIS && P->algorithm != MCG_WOLFF) return fail("unknown algorithm %d", P->algorithm);
+  hipStream_t s = (hipStream_t)stream;
+  DISPATCH(run_noalloc_typed, B, P, s);
 }
 
 extern "C" int mcg_sweeps(const mcg_batch* B, const mcg_plan* P, int64_t sweep0, int32_t nsweeps, void* stream) {
