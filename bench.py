@@ -42,6 +42,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -101,6 +103,17 @@ def cpu_baseline(target_s: float = 10.0):
                       "xyLib.c MCMainFunction (random-sequential Metropolis, glibc rand, full observables), one "
                       "process per point; %.1f s wall" % (P, nsweep, wall),
             "per_core": tot / sum(r[1] for r in res)}
+
+
+def _energy_check(b):
+    """The run's full FP64 recomputes: their period, and the largest
+    |E_tracked - E_full| any replica showed (absolute, and per |E|)."""
+    from mcsolver_amd import _lib as L
+    st = b.state_data()
+    d = st[:, L.S_DRIFT]
+    rel = d / np.maximum(np.abs(st[:, L.S_E]), 1e-300)
+    return {"recompute_every": int(b.b.recompute_every), "max_abs_drift": float(d.max()),
+            "max_rel_drift": float(rel.max()), "error_flag": b.error_flag()}
 
 
 def tuned_options(a, b, pt_every):
@@ -214,13 +227,14 @@ def main():
     acc = b.accumulators()
     assert b.error_flag() == 0
     assert (acc[:, L.A_COUNT] == K + Wm + KT).all(), "every temperature row must have one frame per sweep"
+    energy_check = _energy_check(b)
     out = {"metric": METRIC, "value": value, "unit": "spin updates/s", "n_gpus": 1, "steps": K, "warmup": Wm,
            "ms_per_step": ms / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32", "data": "synthetic",
            "config": {"workload": "XY Binder scan, 7 square lattices x 64-rung PT ladders", "lattices": list(W.XY_SCAN_SIZES),
                       "rungs_per_ladder": 64, "replicas": len(b.reps), "spins_per_sweep": ups, "pt_every": 1,
                       "measure_every": 1, "spin_storage": "fp32", "accumulators": "fp64"},
-           "roofline": roof, "cpu_baseline": cpu}
+           "roofline": roof, "cpu_baseline": cpu, "energy_check": energy_check}
     if a.autotune:
         out["plan"] = opts
     if not a.no_plain:
@@ -284,6 +298,7 @@ def other_workload(a):
     acc = b.accumulators()
     assert b.error_flag() == 0
     assert (acc[:, L.A_COUNT] == K + Wm + KT).all()
+    energy_check = _energy_check(b)
     out = {"metric": METRIC, "value": value, "unit": "spin updates/s", "n_gpus": 1, "steps": K, "warmup": Wm,
            "ms_per_step": ms / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": {"f32": "f32", "f16": "f32 (f16 storage)", "i8": "f32 (int8 spins)"}[wl.store],
@@ -294,7 +309,7 @@ def other_workload(a):
                         "frac": achieved / HBM_PEAK, "traffic": None, "kernel": kern,
                         "avg_launch_us": 1e3 * k_ms / max(k_launch, 1), "launches": int(k_launch),
                         "algorithmic_bytes_per_launch": bpu * k_upd / max(k_launch, 1)},
-           "cpu_baseline": None}
+           "cpu_baseline": None, "energy_check": energy_check}
     if a.autotune:
         out["plan"] = opts
     print(json.dumps(out), flush=True)

@@ -58,6 +58,20 @@ enum { MCG_S_E = 0, MCG_S_M = 1 /* 3 */, MCG_S_EREF = 4, MCG_S_DRIFT = 5, MCG_S_
 /* per-bin accumulators: e, e^2, M, M^2, M^4 */
 enum { MCG_NBINQ = 5 };
 
+/* bits of the device error flag (batch->err), raised by the kernels:
+ * BARRIER    a grid barrier of a persistent launch timed out (the launch left)
+ * NONFINITE  a full FP64 recompute found a non-finite energy or magnetisation
+ *            (NaN / Inf in the spins)
+ * LENGTH     a full FP64 recompute found the mean relative spin-length
+ *            deviation | |s|^2 / S^2 - 1 | above 1e-4 (FP32 storage) or 1e-2
+ *            (FP16 storage)
+ * DRIFT      a full FP64 recompute found |E_tracked - E_full| above 1e-4 n
+ *            (FP32 storage, Ising) or 1e-2 n (FP16 storage), n sites: the
+ *            spins changed outside the sweeps
+ * The full recompute runs every batch->recompute_every sweeps; it also
+ * records the largest |E_tracked - E_full| in MCG_S_DRIFT. */
+enum { MCG_ERR_BARRIER = 16, MCG_ERR_NONFINITE = 32, MCG_ERR_LENGTH = 64, MCG_ERR_DRIFT = 128 };
+
 /*
  * One lattice geometry ("graph"): the per-site link tables of
  * mcMain.py:60-78 / 162-174 compiled for the device.  All pointers are device

@@ -23,6 +23,10 @@ S_E, S_M, S_EREF, S_DRIFT, S_ACC, S_TRY = 0, 1, 4, 5, 6, 7
 NSTATE = 8
 NBINQ = 5
 KERNEL_AUTO, KERNEL_LDS, KERNEL_STREAM, KERNEL_PLAIN = 0, 1, 2, 3
+ERR_BARRIER, ERR_NONFINITE, ERR_LENGTH, ERR_DRIFT = 16, 32, 64, 128
+# production period of the full FP64 recompute (tracked-energy drift, NaN /
+# Inf and spin-length checks): a recompute costs about one sweep
+RECOMPUTE_EVERY = 1000
 
 P = C.c_void_p
 

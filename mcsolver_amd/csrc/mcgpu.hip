@@ -936,10 +936,13 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
   if (!defer) flush_frame<NC>(C, B, row, flags);
 }
 
-// full FP64 recomputation of E and M of one replica (group-parallel)
+// full FP64 recomputation of one replica (group-parallel): out[0] E,
+// out[1..3] M, out[4] the summed relative spin-length deviation
+// sum_i | |s_i|^2 / S_i^2 - 1 | (0 for Ising)
+static constexpr int NREC = 5;
 template <int NC, typename SV>
 __device__ void recompute(const SV& sp, const mcg_graph& G, const float* coup, const float* clsdat, float h,
-                          double* s_red, int g, int lt, int TPR, double* out4) {
+                          double* s_red, int g, int lt, int TPR, double* out) {
   double v[RK];
   for (int k = 0; k < RK; ++k) v[k] = 0.0;
   for (int i = lt; i < G.n; i += TPR) {
@@ -949,10 +952,35 @@ __device__ void recompute(const SV& sp, const mcg_graph& G, const float* coup, c
     v[1] += s[0];
     v[2] += s[1];
     v[3] += s[2];
+    if constexpr (NC > 1) {
+      const double l2 = (double)s[0] * s[0] + (double)s[1] * s[1] + (double)s[2] * s[2];
+      v[4] += fabs(fma(l2, (double)clsdat[G.cls[i] * 8 + 4], -1.0));
+    }
   }
-  group_sum<4>(v, s_red, g, TPR);
+  group_sum<NREC>(v, s_red, g, TPR);
   if (lt == 0)
-    for (int k = 0; k < 4; ++k) out4[k] = v[k];
+    for (int k = 0; k < NREC; ++k) out[k] = v[k];
+}
+
+// Health checks on a recompute (called by the thread holding its result;
+// drift = |E_tracked - E_full|): NaN / Inf in the spins makes E or M
+// non-finite; spin lengths whose mean relative deviation exceeds the
+// storage's tolerance (FP32 keeps |s|^2 to ~1e-7 by the renormalisation,
+// FP16 rounds each component to 11 bits) mean the renormalisation no longer
+// holds; a drift above the storage's tolerance per site (FP32 tracking
+// wanders by ~1e-9 per site per sweep, FP16 by the untracked rounding of
+// stored spins) means the spins changed behind the sweep's back.  Each
+// raises a bit of the device error flag.
+template <typename ST>
+__device__ __forceinline__ void recompute_checks(const mcg_batch& B, const mcg_graph& G, const double* r,
+                                                 double drift) {
+  const bool f16 = sizeof(ST) == 2;
+  if (!isfinite(r[0] + r[1] + r[2] + r[3] + r[4])) {
+    atomicOr(B.err, MCG_ERR_NONFINITE);
+    return;
+  }
+  if (r[4] > (f16 ? 1e-2 : 1e-4) * G.n) atomicOr(B.err, MCG_ERR_LENGTH);
+  if (drift > (f16 ? 1e-2 : 1e-4) * G.n) atomicOr(B.err, MCG_ERR_DRIFT);
 }
 
 // ---------------------------------------------------------------------------
@@ -1137,7 +1165,7 @@ __device__ __forceinline__ bool grid_wait(uint32_t* bar, uint32_t epoch, int32_t
     uint32_t it = 0;
     while ((int32_t)(dev_load(rel) - epoch) < 0) {
       if (++it > (1u << 22)) {
-        atomicOr(err, 16);
+        atomicOr(err, MCG_ERR_BARRIER);
         ok = 0;
         break;
       }
@@ -1226,7 +1254,7 @@ __device__ __forceinline__ void tail_rest(const SV& sv, const mcg_graph& G, cons
                                           bool therm_end, bool rec, uint32_t key0, uint32_t key1,
                                           double mprev = 0.0, bool have_prev = false, bool defer = false) {
   if (rec) {
-    double r4[4];
+    double r4[NREC];
     recompute<NC>(sv, G, s_coup, s_cls, h, s_red, g, lt, TPR, r4);
     if (lt == 0) {
       const double d = fabs(st[MCG_S_E] - r4[0]);
@@ -1235,6 +1263,7 @@ __device__ __forceinline__ void tail_rest(const SV& sv, const mcg_graph& G, cons
       st[MCG_S_M] = r4[1];
       st[MCG_S_M + 1] = r4[2];
       st[MCG_S_M + 2] = r4[3];
+      recompute_checks<ST>(B, G, r4, d);
     }
   }
   if (therm_end && lt == 0) st[MCG_S_EREF] = st[MCG_S_E];
@@ -1756,11 +1785,12 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
         tail_wave0<NC>(B, s_red, s_st, R, TPR, fE, fM, therm_end && !rec, rep_l,
                        rec ? nullptr : a.eslot + (size_t)par * nrepb * MAX_PARTS, s_frm, pl.rem);
         if (rec) {
-          double r4[4];
+          double r4[NREC];
           recompute<NC>(sp, G, C.coup, C.clsdat, hg, s_red, g, lt, TPR, r4);
           if (lt == 0) {
             const double d = fabs(st[MCG_S_E] - r4[0]);
             if (d > st[MCG_S_DRIFT]) st[MCG_S_DRIFT] = d;
+            recompute_checks<ST>(B, G, r4, d);
             for (int q = 0; q < 4; ++q) st[MCG_S_E + q] = r4[q];
             if (therm_end) st[MCG_S_EREF] = st[MCG_S_E];
             dev_store(es, st[MCG_S_E]);
@@ -2060,13 +2090,15 @@ __global__ __launch_bounds__(1024) void k_measure(mcg_batch B, const int32_t* __
   if (threadIdx.x < MCG_NSTATE) st[threadIdx.x] = B.state[(size_t)rep * MCG_NSTATE + threadIdx.x];
   __syncthreads();
   if (mode & 1) {
-    double r4[4];
+    double r4[NREC];
     recompute<NC>(sp, G, G.coup, G.clsdat, h, s_red, 0, threadIdx.x, blockDim.x, r4);
     if (threadIdx.x == 0) {
+      double d = 0.0;
       if (!(mode & 8)) {
-        const double d = fabs(st[MCG_S_E] - r4[0]);
+        d = fabs(st[MCG_S_E] - r4[0]);
         if (d > st[MCG_S_DRIFT]) st[MCG_S_DRIFT] = d;
       }
+      recompute_checks<ST>(B, G, r4, d);
       st[MCG_S_E] = r4[0];
       st[MCG_S_M] = r4[1];
       st[MCG_S_M + 1] = r4[2];

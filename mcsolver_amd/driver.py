@@ -36,7 +36,7 @@ import numpy as np
 
 from . import _lib as L
 from . import lattice as lat
-from .dropin import (DEFAULT_SEED, _plan_counts, _run_frames, ising_couplings, ising_meas_flags, ising_observables,
+from .dropin import (DEFAULT_SEED, _plan_counts, _run_frames, check_health, recompute_period, ising_couplings, ising_meas_flags, ising_observables,
                      on_initial_spins, on_meas_flags, on_observables)
 from .engine import Batch, ReplicaSpec, compile_graph
 
@@ -218,7 +218,7 @@ def build_input_lattice(p: McInput, dipole_cutoff: Optional[float] = None) -> la
     Lt = lat.build_lattice(p.LPack[0], p.LPack[1], p.LPack[2], p.LMatrix, p.pos, _bonds(p.bondList, ising), p.S,
                            p.DList, ki_s=p.GcOrb[0][0], ki_t=p.GcOrb[0][1], ki_overLat=tuple(p.GcOrb[1]),
                            orbGroupList=p.orbGroupList, groupInSC=p.groupInSC,
-                           localCircuitList=p.localCircuitList, ising=ising)
+                           localCircuitList=p.localCircuitList, ising=ising, numpy_T=True)
     if abs(p.dipoleAlpha) > 1e-5:
         Lt = lat.dipolar_bonds(Lt, p.dipoleAlpha, cutoff=dipole_cutoff, ising=ising)
     return Lt
@@ -303,7 +303,8 @@ def run_points(Lt: lat.Lattice, nc: int, algorithm: str, points: Sequence[tuple]
     else:
         reps = [ReplicaSpec(gidx[k], k) for k in range(len(points))]
     b = Batch(nc, graphs, reps, [1.0 / t for t in Tc], [float(H) for _, H in points], init, store=store, meas=meas,
-              nbins=1, nframes=int(nsweep), seed=seed, wolff=wolff)
+              nbins=1, nframes=int(nsweep), seed=seed, wolff=wolff,
+              recompute_every=recompute_period(Lt.nbr, Lt.J, nc == 1))
     b.init()
     nth, spf, ncl = _plan_counts(n, int(nthermal), int(nsweep), ninterval, wolff)
     alg = L.WOLFF if wolff else L.METROPOLIS
@@ -316,8 +317,7 @@ def run_points(Lt: lat.Lattice, nc: int, algorithm: str, points: Sequence[tuple]
     if tempering:
         # accumulators are per row; row order == point order here
         pass
-    if b.error_flag():
-        raise RuntimeError("engine reported error flag %d" % b.error_flag())
+    check_health(b)
     out = []
     for k, (T, H) in enumerate(points):
         t = Tc[k]

@@ -31,7 +31,7 @@ from typing import Optional
 import numpy as np
 
 from . import _lib as L
-from .engine import Batch, ReplicaSpec, compile_graph
+from .engine import Batch, ReplicaSpec, compile_graph, reciprocal_couplings
 
 DEFAULT_SEED = 0x5EED2026
 
@@ -75,6 +75,27 @@ def _run_frames(b: Batch, nth: int, nsweep: int, spf: int, alg: int, ncl: int, s
             for k, r in enumerate(replicas):
                 frames[k].append(b.replica_spins(r).copy())
     return frames
+
+
+def recompute_period(nbr, J, ising: bool, diag_only: bool = False) -> int:
+    """Period of the full FP64 recompute for a run of these couplings: the
+    production period when the couplings are reciprocal, 0 (no recompute)
+    otherwise -- then no energy function has the tracked changes, and the
+    reference's incrementally tracked energy is the quantity it reports."""
+    return L.RECOMPUTE_EVERY if reciprocal_couplings(nbr, J, ising, diag_only) else 0
+
+
+def check_health(b: Batch) -> None:
+    """Raise if the engine's device error flag is set: a persistent launch's
+    barrier timed out, or a full FP64 recompute found NaN / Inf in the spins
+    or spin lengths off their S (include/mcgpu.h MCG_ERR_*)."""
+    f = b.error_flag()
+    if f:
+        what = [nm for bit, nm in ((L.ERR_BARRIER, "grid barrier timed out"),
+                                   (L.ERR_NONFINITE, "non-finite energy or magnetisation (NaN/Inf spins)"),
+                                   (L.ERR_LENGTH, "spin lengths off their S"),
+                                   (L.ERR_DRIFT, "tracked energy off the full recompute")) if f & bit]
+        raise RuntimeError("engine error flag 0x%x: %s" % (f, "; ".join(what) or "unknown"))
 
 
 def ising_couplings(linkStrength, n: int, maxl: int, nlink, per_link_couplings: bool = False) -> np.ndarray:
@@ -181,9 +202,10 @@ def ising_main(algorithm, initSpin, nthermal, nsweep, ninterval, maxNLinking, nl
     meas = ising_meas_flags(nr > 0, wolff)
     # sigma = +1 everywhere: the spin of site i is sigma_i * initSpin[i]
     b = Batch(1, [g], [ReplicaSpec(0, 0)], [1.0], [float(h)], [np.ones(n)], meas=meas, nbins=1,
-              nframes=int(nsweep), seed=seed, wolff=wolff)
+              nframes=int(nsweep), seed=seed, wolff=wolff, recompute_every=recompute_period(nbr, Jeff, True))
     b.init()
     frames = _run_frames(b, nth, int(nsweep), spf, alg, ncl, int(spinFrame))[0]
+    check_health(b)
     out = ising_observables(b.accumulators()[0], int(nsweep), S, int(ninterval), wolff, len(pairs))
     if int(spinFrame) > 0:
         fr = tuple(tuple(float(v) for v in (f[:, 0] * S)) for f in frames)
@@ -240,9 +262,11 @@ def on_main(nc: int, algorithm, initSpin, initD, nthermal, nsweep, ninterval, ma
     nth, spf, ncl = _plan_counts(n, int(nthermal), int(nsweep), int(ninterval), wolff)
     meas = on_meas_flags(nc, circ is not None and len(circ) > 0, nr > 0, ng > 0)
     b = Batch(nc, [g], [ReplicaSpec(0, 0)], [1.0], [float(h)], [init], store=store, meas=meas, nbins=1,
-              nframes=int(nsweep), seed=seed, wolff=wolff)
+              nframes=int(nsweep), seed=seed, wolff=wolff,
+              recompute_every=recompute_period(nbr, J, False, int(ignoreNonDiagonalJ) > 0))
     b.init()
     frames = _run_frames(b, nth, int(nsweep), spf, alg, ncl, int(spinFrame))[0]
+    check_health(b)
     vals = on_observables(b.accumulators()[0], int(nsweep), nc, nr > 0)
     if int(spinFrame) > 0:
         fr = tuple(tuple((float(v[0]), float(v[1]), float(v[2]) if nc == 3 else 0.0) for v in f) for f in frames)

@@ -69,6 +69,45 @@ class GraphTables:
         return len(self.col_ptr) - 1
 
 
+# transpose in the reference tensor order [xx,yy,zz,xy,xz,yz,yx,zx,zy]
+_T9 = np.array([0, 1, 2, 6, 7, 8, 3, 4, 5])
+
+
+def reciprocal_couplings(nbr: np.ndarray, J: np.ndarray, ising: bool, diag_only: bool = False,
+                         rtol: float = 1e-9) -> bool:
+    """True when every coupling has its reciprocal: the summed tensor of the
+    links i -> j equals the transpose of the summed tensor of the links
+    j -> i (Ising: the same number).  Only then do the single-site updates
+    change one energy function, which the full FP64 recompute can check the
+    tracked energy against.  Lattices built from a bond list are reciprocal
+    (Lattice.py:256-262: the target gets J^T); tuples read with the Ising
+    linkStrength + i convention (isingLib.c:30) and unequal couplings, or
+    in-place merged tensors of tiny lattices, are not."""
+    nbr = np.asarray(nbr, np.int64)
+    n, z = nbr.shape
+    if z == 0 or n == 0:
+        return True
+    Jv = np.asarray(J, np.float64).reshape(n, z, -1)
+    Jv = Jv[..., :1] if ising else Jv[..., :9].copy()
+    if diag_only and not ising:
+        Jv[..., 3:] = 0.0
+    i = np.repeat(np.arange(n), z)
+    j = nbr.reshape(-1)
+    m = j >= 0
+    i, j, v = i[m], j[m], Jv.reshape(n * z, -1)[m]
+    key = i * n + j
+    uk, inv = np.unique(key, return_inverse=True)
+    sums = np.zeros((len(uk), v.shape[1]))
+    np.add.at(sums, inv, v)
+    rk = (uk % n) * n + uk // n
+    pos = np.minimum(np.searchsorted(uk, rk), len(uk) - 1)
+    if not np.all(uk[pos] == rk):
+        return False
+    rev = sums[pos] if ising else sums[pos][:, _T9]
+    scale = max(float(np.abs(sums).max()), 1e-300)
+    return bool(np.all(np.abs(sums - rev) <= rtol * scale))
+
+
 def compile_graph(nc: int, nbr: np.ndarray, J: np.ndarray, S: np.ndarray, D: np.ndarray,
                   pairs: Optional[np.ndarray] = None, circuits: Optional[np.ndarray] = None,
                   rOrb: Optional[np.ndarray] = None, rOrbCluster: Optional[np.ndarray] = None,
@@ -498,3 +537,34 @@ class Batch:
     def error_flag(self) -> int:
         torch.cuda.synchronize()
         return int(self.err.item())
+
+    # ---------------------------------------------------------------------
+    _CKPT = ("spins", "state", "acc", "bins", "gacc", "pt_acc", "rung", "who", "err")
+
+    def checkpoint(self) -> dict:
+        """Everything a run continues from, in host memory: spins, tracked
+        state, accumulators, bins, group sums, ladder state (rung, who, swap
+        counts), the error flag, the next Philox sweep index and the Wolff
+        cluster stamps.  Philox is counter based, so a run restored from it
+        (restore, then run with sweep0 = next_sweep) continues the trajectory
+        bit for bit, in this process or another."""
+        torch.cuda.synchronize()
+        ck = {k: getattr(self, k).cpu().numpy().copy() for k in self._CKPT}
+        ck["next_sweep"] = np.int64(self._next_sweep)
+        ck["wolff_cl_end"] = np.int64(self._wolff_cl_end)
+        if self.wstamp is not None:
+            ck["wstamp"] = self.wstamp.cpu().numpy().copy()
+        return ck
+
+    def restore(self, ck: dict) -> None:
+        """Load a checkpoint taken from a batch of the same shape."""
+        names = list(self._CKPT) + (["wstamp"] if self.wstamp is not None else [])
+        for k in names:
+            t = getattr(self, k)
+            a = np.asarray(ck[k])
+            if a.shape != tuple(t.shape) or a.dtype != t.cpu().numpy().dtype:
+                raise ValueError("checkpoint %s: shape %s / %s does not match the batch" % (k, a.shape, a.dtype))
+            t.copy_(torch.from_numpy(a).to(t.device))
+        self._next_sweep = int(ck["next_sweep"])
+        self._wolff_cl_end = int(ck["wolff_cl_end"])
+        torch.cuda.synchronize()

@@ -26,10 +26,17 @@ Restates, vectorised with numpy, what ``mcsolver/Lattice.py`` builds and how
   groups with -1                                         (mcMain.py:60-98,
                                                           150-221)
 
-One departure: when a duplicate link is merged, the reference adds the new
-tensor into the existing one in place, and that array object is the bond's
-own ``strength`` shared by every site (Lattice.py:51 aliasing); here only the
-one link is changed.  Duplicates arise only for L <= 2*max|overLat|.
+Repeated targets (only for L <= 2*max|overLat|) follow the reference's
+object semantics exactly (reference_links): a link holds a reference to its
+bond's scaled tensor; a merge adds the new tensor into that array *in place*
+(Lattice.py:51), so every link sharing the array -- the bond's own strength
+or invStrength -- carries the sum, and later skip tests see it; a float
+(Ising) merge rebinds only that link.  The scaled values are built in the
+reference's order (each bond scaled by 1/T, then added), and the Ising skip
+test applies only to Python floats (Lattice.py:45: `type(strength) is
+float`), i.e. not when T is a numpy scalar, as in loadMC's np.linspace
+temperatures.  Lattices with repeats keep their bonds (link_bonds) so that
+ising_args / on_args redo this at each T.
 """
 from __future__ import annotations
 
@@ -96,6 +103,8 @@ class Lattice:
     pos: np.ndarray              # [n, 3] cartesian positions
     cell: np.ndarray = field(default=None)  # [n, 3] cell coords
     orb: np.ndarray = field(default=None)   # [n] orbital index
+    link_bonds: Optional[list] = field(default=None)  # bonds, kept when some site links one target twice
+    ising: bool = False
 
     @property
     def maxl(self) -> int:
@@ -115,12 +124,83 @@ def _group_ordered(site: np.ndarray, other: np.ndarray, seq: np.ndarray, n: int)
     return order, np.bincount(site, minlength=n)
 
 
+def reference_links(Lx: int, Ly: int, Lz: int, norb: int, bonds: Sequence[Bond], T, ising: bool):
+    """The normal links of Lattice.establishLinking (Lattice.py:237-262) with
+    the object semantics of Orbital.addLinking (Lattice.py:34-57), for a
+    temperature T (the bonds scaled by (1/T) first, Lattice.py:141-147, with
+    T as given: a numpy T makes Ising strengths numpy scalars).
+
+    Returns (nbr [n, maxl] -1 padded, vals [n, maxl] list of float or 9-arrays
+    as the flattening reads them after all merges).  Pure Python: used only
+    for lattices where a site links one target twice."""
+    n = Lx * Ly * Lz * norb
+    objs = []   # float / numpy scalar (immutable) or ndarray (mutable, shared)
+    s_ref, i_ref = [], []
+    for b in bonds:
+        j = b.j9()
+        if ising:
+            jx = float(j[0])
+            objs.append((1 / T) * jx)
+            s_ref.append(len(objs) - 1)
+            objs.append((1 / T) * jx)
+            i_ref.append(len(objs) - 1)
+        else:
+            objs.append((1 / T) * np.array([float(v) for v in j]))
+            s_ref.append(len(objs) - 1)
+            objs.append((1 / T) * np.array([float(v) for v in j[J_TRANSPOSE]]))
+            i_ref.append(len(objs) - 1)
+    links = [[] for _ in range(n)]
+
+    def add(site, target, oi):
+        new = objs[oi]
+        for lk in links[site]:
+            if lk[0] != target:
+                continue
+            old = objs[lk[1]]
+            d = abs(old - new)
+            if type(new) is float and d < 1e-5:
+                return
+            if type(new) is np.ndarray and sum(d) < 1e-5:
+                return
+            if isinstance(old, np.ndarray):
+                old += new          # in place: every link holding this array sees the sum
+            else:
+                objs.append(old + new)
+                lk[1] = len(objs) - 1
+            return
+        links[site].append([target, oi])
+
+    for x in range(Lx):
+        for y in range(Ly):
+            for z in range(Lz):
+                for o in range(norb):
+                    src = ((x * Ly + y) * Lz + z) * norb + o
+                    for bi, b in enumerate(bonds):
+                        if o != b.source:
+                            continue
+                        dx, dy, dz = (int(v) for v in b.overLat)
+                        tgt = ((((x + dx) % Lx) * Ly + (y + dy) % Ly) * Lz + (z + dz) % Lz) * norb + b.target
+                        add(src, tgt, s_ref[bi])
+                        if src != tgt:
+                            add(tgt, src, i_ref[bi])
+    maxl = max((len(lk) for lk in links), default=0)
+    nbr = -np.ones((n, maxl), np.int32)
+    vals = [[None] * maxl for _ in range(n)]
+    for i, lk in enumerate(links):
+        for l, (t, oi) in enumerate(lk):
+            nbr[i, l] = t
+            vals[i][l] = objs[oi]
+    return nbr, vals
+
+
 def build_lattice(Lx: int, Ly: int, Lz: int, LMatrix, pos, bonds: Sequence[Bond], S: Sequence[float],
                   D: Optional[Sequence[Sequence[float]]] = None, ki_s: int = 0, ki_t: int = 0,
                   ki_overLat=(0, 0, 0), orbGroupList: Sequence[Sequence[int]] = (), groupInSC: bool = False,
-                  localCircuitList: Sequence = (), ising: bool = False) -> Lattice:
+                  localCircuitList: Sequence = (), ising: bool = False, numpy_T: bool = False) -> Lattice:
     """Build the per-site tables the way Lattice.establishLattice +
-    establishLinking do (see module docstring for the rules and citations)."""
+    establishLinking do (see module docstring for the rules and citations).
+    numpy_T: repeated Ising links merge as they do when the reference's T is a
+    numpy scalar (loadMC), instead of as for a Python float."""
     LMatrix = np.asarray(LMatrix, dtype=np.float64)
     pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
     norb = len(S)
@@ -169,23 +249,33 @@ def build_lattice(Lx: int, Ly: int, Lz: int, LMatrix, pos, bonds: Sequence[Bond]
     dup = np.ones(len(key), bool)
     dup[first_idx] = False
     keep = ~dup
-    for k in np.nonzero(dup)[0]:
-        f = first_idx[inv[k]]
-        diff = np.abs(jj[f] - jj[k])
-        if (diff[0] if ising else diff.sum()) < 1e-5:
-            continue
-        jj[f] += jj[k]
-    site, other, jj = site[keep], other[keep], jj[keep]
-    nlink = np.bincount(site, minlength=n).astype(np.int32)
-    maxl = int(nlink.max()) if n else 0
-    start = np.concatenate([[0], np.cumsum(nlink)[:-1]])
-    slot = np.arange(len(site)) - start[site]
-    nbr = -np.ones((n, max(maxl, 1)), np.int32)
-    J = np.zeros((n, max(maxl, 1), 9))
-    nbr[site, slot] = other
-    J[site, slot] = jj
-    if maxl == 0:
-        nbr = nbr[:, :0]; J = J[:, :0]
+    repeats = bool(dup.any())
+    if repeats:
+        # some site links one target twice: the reference's merge / skip with
+        # its in-place tensor additions, unscaled (T = 1)
+        nbr, vals = reference_links(Lx, Ly, Lz, norb, bonds, np.float64(1.0) if numpy_T else 1.0, ising)
+        maxl = nbr.shape[1]
+        nlink = (nbr >= 0).sum(1).astype(np.int32)
+        J = np.zeros((n, maxl, 9))
+        for i in range(n):
+            for l in range(int(nlink[i])):
+                v = vals[i][l]
+                if ising:
+                    J[i, l, 0] = float(v)
+                else:
+                    J[i, l] = v
+    else:
+        site, other, jj = site[keep], other[keep], jj[keep]
+        nlink = np.bincount(site, minlength=n).astype(np.int32)
+        maxl = int(nlink.max()) if n else 0
+        start = np.concatenate([[0], np.cumsum(nlink)[:-1]])
+        slot = np.arange(len(site)) - start[site]
+        nbr = -np.ones((n, max(maxl, 1)), np.int32)
+        J = np.zeros((n, max(maxl, 1), 9))
+        nbr[site, slot] = other
+        J[site, slot] = jj
+        if maxl == 0:
+            nbr = nbr[:, :0]; J = J[:, :0]
 
     # ---- block-spin sites: chosen cells (all coords even), Lattice.py:185-205
     chosen_cell = (cx % 2 + cy % 2 + cz % 2) == 0
@@ -258,7 +348,8 @@ def build_lattice(Lx: int, Ly: int, Lz: int, LMatrix, pos, bonds: Sequence[Bond]
     cellv = np.repeat(cxyz, norb, 0)
     posv = (cellv + pos[orbv]) @ LMatrix
     return Lattice(Lx, Ly, Lz, norb, n, nlink, nbr, J, Sarr, Darr, corr, rOrb.astype(np.int32),
-                   clu.astype(np.int32), rlink, circuits, groups, posv, cellv, orbv)
+                   clu.astype(np.int32), rlink, circuits, groups, posv, cellv, orbv,
+                   link_bonds=list(bonds) if repeats else None, ising=ising)
 
 
 def dipolar_bonds(lat: Lattice, alpha: float, cutoff: Optional[float] = None, ising: bool = False,
@@ -326,6 +417,7 @@ def dipolar_bonds(lat: Lattice, alpha: float, cutoff: Optional[float] = None, is
     rlink = -np.ones((lat.rlink.shape[0], maxl), np.int32)
     rlink[:, :lat.rlink.shape[1]] = lat.rlink
     out = Lattice(**{k: getattr(lat, k) for k in lat.__dataclass_fields__})
+    out.link_bonds = None  # the dipolar links are not in the bond list
     out.nlink, out.nbr, out.J, out.rlink = nlink.astype(np.int32), nbr, J, rlink
     return out
 
@@ -340,15 +432,49 @@ def ising_args(lat: Lattice, T: float, h: float, nthermal: int, nsweep: int, nin
     """Positional arguments of isinglib.MCMainFunction exactly as
     MC.mainLoopViaCLib builds them (mcMain.py:51-124)."""
     T = clamp_T(T)
-    n, maxl = lat.n, lat.maxl
+    n = lat.n
     ninterval = n if ninterval <= 0 else ninterval
-    Js = (1 / T) * lat.J[:, :, 0]
-    Js = np.where(lat.nbr >= 0, Js, 0.0)
+    nbr, nlink, flat = _scaled_links(lat, T, True)
     return (1 if algorithm == "Wolff" else 0, tuple(float(v) for v in lat.S), int(nthermal), int(nsweep),
-            int(ninterval), maxl, tuple(int(v) for v in lat.nlink), tuple(float(v) for v in Js.reshape(-1)),
-            tuple(int(v) for v in lat.nbr.reshape(-1)), tuple(int(v) for v in lat.corr_pairs.reshape(-1)),
+            int(ninterval), nbr.shape[1], tuple(int(v) for v in nlink), flat,
+            tuple(int(v) for v in nbr.reshape(-1)), tuple(int(v) for v in lat.corr_pairs.reshape(-1)),
             h / T, tuple(int(v) for v in lat.rOrb), tuple(int(v) for v in lat.rOrbCluster.reshape(-1)),
-            tuple(int(v) for v in lat.rlink.reshape(-1)), int(spinFrame), None)
+            tuple(int(v) for v in _pad(lat.rlink, nbr.shape[1]).reshape(-1)), int(spinFrame), None)
+
+
+def _pad(a: np.ndarray, width: int) -> np.ndarray:
+    """-1 padded to `width` columns (maxNLinking; mcMain.py:81-87, 209-215)."""
+    out = -np.ones((a.shape[0], width), np.int32)
+    w = min(width, a.shape[1])
+    out[:, :w] = a[:, :w]
+    return out
+
+
+def _scaled_links(lat: Lattice, T, ising: bool):
+    """(nbr, nlink, flattened beta-scaled couplings) as mcMain.py:60-78 /
+    162-174 read them at temperature T: (1/T)*J per component, or, for a
+    lattice with repeated targets, the reference's merge semantics at T."""
+    if lat.link_bonds is not None:
+        nbr, vals = reference_links(lat.Lx, lat.Ly, lat.Lz, lat.norb, lat.link_bonds, T, ising)
+        nlink = (nbr >= 0).sum(1)
+        flat = []
+        for i in range(lat.n):
+            for l in range(nbr.shape[1]):
+                v = vals[i][l]
+                if ising:
+                    flat.append(float(v) if v is not None else 0.0)
+                elif v is None:
+                    flat += [0.0] * 9
+                else:
+                    flat += [float(x) for x in v]
+        return nbr, nlink, tuple(flat)
+    if ising:
+        Js = (1 / T) * lat.J[:, :, 0]
+        Js = np.where(lat.nbr >= 0, Js, 0.0)
+    else:
+        Js = (1 / T) * lat.J
+        Js = np.where((lat.nbr >= 0)[:, :, None], Js, 0.0)
+    return lat.nbr, lat.nlink, tuple(float(v) for v in Js.reshape(-1))
 
 
 def on_args(lat: Lattice, T: float, h: float, nthermal: int, nsweep: int, ninterval: int = -1,
@@ -356,11 +482,12 @@ def on_args(lat: Lattice, T: float, h: float, nthermal: int, nsweep: int, ninter
     """Positional arguments of {xylib,heisenberglib}.MCMainFunction exactly
     as MC.mainLoopViaCLib_On builds them (mcMain.py:137-248)."""
     T = clamp_T(T)
-    n, maxl = lat.n, lat.maxl
+    n = lat.n
     ninterval = n if ninterval <= 0 else ninterval
-    Js = (1 / T) * lat.J
-    Js = np.where((lat.nbr >= 0)[:, :, None], Js, 0.0)
-    valid = lat.nbr >= 0
+    nbr, nlink, flat = _scaled_links(lat, T, False)
+    maxl = nbr.shape[1]
+    Js = np.asarray(flat, np.float64).reshape(n, maxl, 9)
+    valid = nbr >= 0
     ignoreNonDiagonalJ = 0 if np.any(np.abs(Js[valid][:, 3:]) > 1e-6) else 1
     DT = lat.D / T
     ng = len(lat.groups)
@@ -373,11 +500,11 @@ def on_args(lat: Lattice, T: float, h: float, nthermal: int, nsweep: int, ninter
     for g in lat.groups:
         glist += list(g) + [-1] * (maxg - len(g))
     return (1 if algorithm == "Wolff" else 0, tuple(float(v) for v in lat.S), tuple(float(v) for v in DT.reshape(-1)),
-            int(nthermal), int(nsweep), int(ninterval), maxl, tuple(int(v) for v in lat.nlink),
-            tuple(float(v) for v in Js.reshape(-1)), tuple(int(v) for v in lat.nbr.reshape(-1)),
+            int(nthermal), int(nsweep), int(ninterval), maxl, tuple(int(v) for v in nlink), flat,
+            tuple(int(v) for v in nbr.reshape(-1)),
             tuple(int(v) for v in lat.circuits.reshape(-1)), tuple(int(v) for v in lat.corr_pairs.reshape(-1)),
             ng, maxg, tuple(glist), float(flunc), h / T, tuple(int(v) for v in lat.rOrb),
-            tuple(int(v) for v in lat.rOrbCluster.reshape(-1)), tuple(int(v) for v in lat.rlink.reshape(-1)),
+            tuple(int(v) for v in lat.rOrbCluster.reshape(-1)), tuple(int(v) for v in _pad(lat.rlink, maxl).reshape(-1)),
             int(spinFrame), ignoreNonDiagonalJ, None)
 
 

@@ -96,8 +96,12 @@ def _init(nc, n, S, mode, rng):
 
 
 def build(name: str, store: str = "f32", nbins: int = 32, nframes: int = 1, meas: int = 0,
-          sizes=XY_SCAN_SIZES, pt: bool = True, nrep_cubic: int = 128, wolff: bool = False, device="cuda") -> Workload:
-    rng = np.random.default_rng(SEED)
+          sizes=XY_SCAN_SIZES, pt: bool = True, nrep_cubic: int = 128, wolff: bool = False, device="cuda",
+          seed: int = SEED, recompute_every: int = L.RECOMPUTE_EVERY) -> Workload:
+    """The named workload; seed keys its Philox streams (and random starts);
+    recompute_every is the period of the full FP64 recompute (drift and
+    spin-health checks, 0 = off)."""
+    rng = np.random.default_rng(seed)
     reps: List[ReplicaSpec] = []
     graphs, lattices, temps, fields, rgraph, inits = [], [], [], [], [], []
     nc = {"ising_square32": 1, "heis_honeycomb64": 3, "xy_ladder_scan": 2, "heis_triangular_dmi128": 3, "heis_cubic64": 3}[name]
@@ -154,7 +158,7 @@ def build(name: str, store: str = "f32", nbins: int = 32, nframes: int = 1, meas
     temps = np.asarray(temps)
     beta = 1.0 / np.maximum(temps, 0.1)  # the reference runs T < 0.1 at 0.1 (mcMain.py:21)
     b = Batch(nc, graphs, reps, beta, fields, inits, store=("i8" if nc == 1 else store), meas=meas, nbins=nbins,
-              nframes=nframes, seed=SEED, wolff=wolff, device=device)
+              nframes=nframes, seed=seed, wolff=wolff, device=device, recompute_every=recompute_every)
     return Workload(name, nc, b, lattices, temps, np.asarray(fields), np.asarray(rgraph),
                     "i8" if nc == 1 else store, pt_every, desc)
 
@@ -171,7 +175,8 @@ def cubic_replicas(T: float, nrep: int = 16, wolff: bool = False, nframes: int =
     inits = [_init(3, Lt.n, Lt.S, "random", rng) for _ in range(nrep)]
     temps = np.full(nrep, float(T))
     b = Batch(3, [_graph(3, Lt)], reps, 1.0 / np.maximum(temps, 0.1), [0.0] * nrep, inits, store=store,
-              nbins=nframes, nframes=nframes, seed=SEED, wolff=wolff, device=device)
+              nbins=nframes, nframes=nframes, seed=SEED, wolff=wolff, device=device,
+              recompute_every=L.RECOMPUTE_EVERY)
     return Workload("heis_cubic64", 3, b, [Lt], temps, np.zeros(nrep), np.zeros(nrep, int), store, 0,
                     "Heisenberg cubic 64^3, J=-1, random start, %d replicas at T=%g" % (nrep, T))
 

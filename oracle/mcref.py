@@ -30,6 +30,9 @@ def lib() -> C.CDLL:
             subprocess.run(["make", "-s", "-C", HERE], check=True)
         _l = C.CDLL(LIB)
         _l.mcref_srand.argtypes = [C.c_uint]
+        _l.mcref_rng_mode.argtypes = [I]
+        _l.mcref_xoshiro_set.argtypes = [C.c_uint64] * 4
+        _l.mcref_xoshiro_next.restype = C.c_uint64
         _l.mcref_ising_run.argtypes = [I, L_, P, L_, L_, L_, I, P, P, P, L_, P, D, L_, L_, P, P, P, I, P, P]
         _l.mcref_on_run.argtypes = [I, I, L_, P, P, L_, L_, L_, I, P, P, P, L_, P, L_, P, L_, L_, P, D, D, L_, L_, P,
                                     P, P, I, I, P, P, P]
@@ -52,6 +55,13 @@ def _a(x, dt):
 
 def ptr(a: np.ndarray):
     return a.ctypes.data_as(P)
+
+
+def rng_mode(mode: str) -> None:
+    """Integer source of the restatement's draws: "glibc" (rand(), the
+    reference's generator, the default) or "xoshiro" (xoshiro256**, top 31
+    bits on rand()'s range, same consumption order)."""
+    lib().mcref_rng_mode({"glibc": 0, "xoshiro": 1}[mode])
 
 
 # ---------------------------------------------------------------- glibc replay

@@ -46,17 +46,62 @@
 
 #define REF_PI 3.1415926535
 
-static double urand(void) { return rand() / (double)RAND_MAX; }
+/* The integer source of every draw.  Mode 0 (default): glibc rand(), the
+ * reference's own generator (replay).  Mode 1: xoshiro256** (Blackman and
+ * Vigna, "Scrambled linear pseudorandom number generators", 2018; state
+ * seeded by SplitMix64 from the srand seed), its top 31 bits, i.e. values
+ * on rand()'s range [0, RAND_MAX = 2^31 - 1], consumed in the same order.
+ * glibc's rand() is an additive lagged-Fibonacci generator with lags 3 and
+ * 31 (r_i = r_{i-3} + r_{i-31}, up to a carry), and one update consumes 3 to
+ * 6 consecutive values; mode 1 removes that generator from the algorithm to
+ * tell its effect on the sampled averages apart from the algorithm's. */
+static int g_rng_mode = 0;
+static uint64_t g_xs[4];
+
+static uint64_t splitmix64(uint64_t *x) {
+  uint64_t z = (*x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+static inline uint64_t rotl64(uint64_t v, int k) { return (v << k) | (v >> (64 - k)); }
+
+static uint64_t xoshiro256ss(void) {
+  const uint64_t out = rotl64(g_xs[1] * 5, 7) * 9;
+  const uint64_t t = g_xs[1] << 17;
+  g_xs[2] ^= g_xs[0];
+  g_xs[3] ^= g_xs[1];
+  g_xs[1] ^= g_xs[2];
+  g_xs[0] ^= g_xs[3];
+  g_xs[2] ^= t;
+  g_xs[3] = rotl64(g_xs[3], 45);
+  return out;
+}
+
+static int ref_rand(void) { return g_rng_mode ? (int)(xoshiro256ss() >> 33) : rand(); }
+
+static double urand(void) { return ref_rand() / (double)RAND_MAX; }
 
 /* isingLib.c:239-241 / heisenbergLib.c:443-445: two draws, r1 first. */
 static long pick_site(long n) {
-  unsigned long long r1 = (unsigned long long)rand();
-  unsigned long long r2 = (unsigned long long)rand();
+  unsigned long long r1 = (unsigned long long)ref_rand();
+  unsigned long long r2 = (unsigned long long)ref_rand();
   return (long)((r1 * RAND_MAX + r2) % (unsigned long long)n);
 }
 
-void mcref_srand(unsigned int seed) { srand(seed); }
-int mcref_rand(void) { return rand(); }
+void mcref_srand(unsigned int seed) {
+  srand(seed);
+  uint64_t x = 0x5EED000000000000ull ^ seed;
+  for (int k = 0; k < 4; ++k) g_xs[k] = splitmix64(&x);
+}
+void mcref_rng_mode(int mode) { g_rng_mode = mode ? 1 : 0; }
+/* known-answer access to the xoshiro256** state (tests) */
+void mcref_xoshiro_set(uint64_t s0, uint64_t s1, uint64_t s2, uint64_t s3) {
+  g_xs[0] = s0; g_xs[1] = s1; g_xs[2] = s2; g_xs[3] = s3;
+}
+uint64_t mcref_xoshiro_next(void) { return xoshiro256ss(); }
+int mcref_rand(void) { return ref_rand(); }
 
 /* ========================================================================== */
 /* Ising                                                                       */
@@ -147,7 +192,11 @@ static double ising_corr_rnorm(const ising_t *m, long site) {
   double c = 0;
   double ms = ising_majority(m, site);
   for (int l = 0; l < m->nlink[site]; l++) {
-    double mt = ising_majority(m, m->rlink[ri * m->maxl + l]);
+    /* a -1 padded block-spin link (small or odd L) is undefined behaviour in
+       the reference; here, as in the engine, an absent link adds nothing */
+    const int t = m->rlink[ri * m->maxl + l];
+    if (t < 0) continue;
+    double mt = ising_majority(m, t);
     c += m->J[site + l] * ms * mt;
   }
   return c;
@@ -620,7 +669,9 @@ int mcref_on_run(int nc, int algorithm, long n, const double *initSpin, const do
       double v[3], w[3], cc = 0;
       on_majority(&m, j, v);
       for (int l = 0; l < m.nlink[j]; l++) {
-        on_majority(&m, m.rlink[ri * m.maxl + l], w);
+        const int t = m.rlink[ri * m.maxl + l];
+        if (t < 0) continue;  /* absent block-spin link: adds nothing (UB in the reference) */
+        on_majority(&m, t, w);
         cc += dot9(&m, v, w, m.J + (j * m.maxl + l) * 9);
       }
       er += cc;

@@ -10,7 +10,11 @@ Metropolis with glibc rand(), FP64) record the per-run moments
 from which the test forms E, C, |M|, chi and U4.  The oracle is our own C
 code, so this runs anywhere the oracle builds; one process per run.
 
-    python tests/golden/make_honeycomb_fp64.py      # writes honeycomb_fp64.json
+Integer draws from xoshiro256** (default, honeycomb_fp64.json) or, with
+--rng=glibc, from glibc rand() as the reference (honeycomb_fp64_glibc.json);
+see make_config_fp64.py for why the parity reference uses the former.
+
+    python tests/golden/make_honeycomb_fp64.py [--rng=xoshiro|glibc]
 """
 import json
 import multiprocessing as mp
@@ -23,9 +27,13 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "honeycomb_fp64.json")
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def out_path(rng):
+    return os.path.join(HERE, "honeycomb_fp64.json" if rng == "xoshiro" else "honeycomb_fp64_%s.json" % rng)
 TEMPS_IDX = (10, 30, 45, 60)      # of linspace(0.1, 2.0, 64), the honeycomb workload's ladder
-NTHERMAL, NSWEEP, R = 500, 2000, 8
+NTHERMAL, NSWEEP, R = 500, 2000, 32
 SEED0 = 31000
 
 
@@ -37,33 +45,36 @@ def lattice():
 
 
 def one(job):
-    T, seed = job
+    T, seed, rng = job
     from mcsolver_amd import lattice as lat
     from oracle import mcref
+    mcref.rng_mode(rng)
     H = lattice()
     a = lat.on_args(H, T, 0.0, NTHERMAL, NSWEEP)
     out, _, g = mcref.on_run(3, a, seed=seed)
     return [float(out[8]), float(out[9]), float(out[22]), float(g[3]), float(g[5])]
 
 
-def main():
+def main(rng="xoshiro"):
     temps = [float(t) for t in np.linspace(0.1, 2.0, 64)[list(TEMPS_IDX)]]
-    jobs = [(T, SEED0 + 100 * i + r) for i, T in enumerate(temps) for r in range(R)]
+    jobs = [(T, SEED0 + 100 * i + r, rng) for i, T in enumerate(temps) for r in range(R)]
     t0 = time.time()
     with mp.get_context("fork").Pool(min(8, os.cpu_count() or 1)) as pool:
         res = pool.map(one, jobs)
     runs = {}
-    for (T, _), v in zip(jobs, res):
+    for (T, _, _), v in zip(jobs, res):
         runs.setdefault("%.6f" % T, []).append(v)
     H = lattice()
     doc = {"lattice": "honeycomb 64x64 cells (8192 sites), full 3x3 tensors, Dz=-0.1, seed 0x5EED2026",
            "nLat": int(H.n // 2), "n": int(H.n), "nthermal": NTHERMAL, "nsweep": NSWEEP, "runs_per_T": R,
            "moments": ["E", "E2", "absM", "M2", "M4"], "temps": temps, "runs": runs,
-           "generator": "oracle/ref_glibc.c mcref_on_run via oracle/mcref.py, FP64"}
+           "generator": "oracle/ref_glibc.c mcref_on_run via oracle/mcref.py, FP64, integer draws from %s" %
+                        ("xoshiro256** (top 31 bits)" if rng == "xoshiro" else "glibc rand()"), "rng": rng}
+    OUT = out_path(rng)
     with open(OUT, "w") as f:
         json.dump(doc, f, indent=1)
     print("wrote %s (%d runs, %.0f s)" % (OUT, len(jobs), time.time() - t0))
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1].split("=", 1)[1] if len(sys.argv) > 1 and sys.argv[1].startswith("--rng=") else "xoshiro")

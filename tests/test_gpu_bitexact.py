@@ -150,6 +150,49 @@ def test_wolff_bitexact(gpu, case):
         assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
+WOLFF_ZERO = {
+    # name: (nc, lattice, betas, h, correction is identically zero)
+    "heis_cubic4_noD_h0": (3, lambda: lat.cubic(4, J=-1.0), [0.6, 1.6], 0.0, True),
+    "xy_sq12_noD_h0": (2, lambda: lat.square(12, J=-1.0), [0.7, 1.4], 0.0, True),
+    "ising_sq16_h0": (1, lambda: lat.square(16, ising=True), [0.3, 0.6], 0.0, True),
+    # isotropic exchange but a single-ion term, or a field: evaluated
+    "heis_cubic4_D_h0": (3, lambda: lat.cubic(4, J=-1.0, D=(0.0, 0.0, -0.4)), [0.6, 1.6], 0.0, False),
+    "heis_cubic4_noD_h": (3, lambda: lat.cubic(4, J=-1.0), [0.6, 1.6], 0.3, False),
+    "xy_sq12_D_h0": (2, lambda: lat.square(12, J=-1.0, D=(0.3, 0.0, 0.0)), [0.7, 1.4], 0.0, False),
+}
+
+
+@pytest.mark.parametrize("case", sorted(WOLFF_ZERO))
+def test_wolff_zero_correction_rule_bitexact(gpu, case):
+    """The Wolff cluster acceptance skips the correction energy exactly when
+    it is identically zero (isotropic exchange, no single-ion term, h = 0;
+    heisenbergLib.c:403-423): engine and checker agree on that rule (spins
+    bit for bit), every cluster is then accepted (S_ACC == S_TRY), and with
+    a single-ion term or a field the correction is evaluated by both."""
+    nc, mk, betas, h, zero = WOLFF_ZERO[case]
+    Lt = mk()
+    print("wolff case", case, flush=True)
+    b, g, init = _mk_batch(nc, Lt, betas, h, wolff=True)
+    ncl = 8
+    b.sweeps(0, 3, algorithm=L.WOLFF, clusters_per_sweep=ncl)
+    got = _gpu_spins(b, len(betas))
+    ref = _cpu_spins(nc, init, "f32")
+    nbr, J, S, D = _lattice_tables(Lt, nc)
+    for k in range(len(betas)):
+        one = np.ascontiguousarray(ref[k])
+        mcref.cb_wolff(nc, 0 if nc > 1 else 2, nbr, J, S, D, g.colour, betas[k], h, k, SEED, 0, 3 * ncl, one)
+        ref[k] = one
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    st = b.state_data()
+    print("accepted / grown spins:", st[:, L.S_ACC], st[:, L.S_TRY])
+    assert (st[:, L.S_TRY] > 0).all()
+    if zero:
+        assert np.array_equal(st[:, L.S_ACC], st[:, L.S_TRY])
+    else:
+        assert (st[:, L.S_ACC] <= st[:, L.S_TRY]).all()
+    assert b.error_flag() == 0
+
+
 @pytest.mark.parametrize("no_persist", [False, True])
 def test_parallel_tempering_bitexact(gpu, no_persist):
     """Ising ladder, swap every sweep: permutation, rungs and spins match

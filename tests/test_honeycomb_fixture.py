@@ -24,5 +24,5 @@ def test_fixture_run_regenerates():
     d = json.load(open(FIX))
     T = d["temps"][0]
     assert (g.NTHERMAL, g.NSWEEP) == (d["nthermal"], d["nsweep"])
-    got = g.one((T, g.SEED0))
+    got = g.one((T, g.SEED0, d["rng"]))
     assert got == d["runs"]["%.6f" % T][0]
