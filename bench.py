@@ -105,6 +105,18 @@ def cpu_baseline(target_s: float = 10.0):
             "per_core": tot / sum(r[1] for r in res)}
 
 
+def copy_calibration(mib: int) -> None:
+    """One device-to-device copy of `mib` MiB: a kernel of known traffic, for
+    scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
+    if mib > 0:
+        import torch
+        src = torch.empty(mib << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+
 def _energy_check(b):
     """The run's full FP64 recomputes: their period, and the largest
     |E_tracked - E_full| any replica showed (absolute, and per |E|)."""
@@ -177,12 +189,7 @@ def main():
     from mcsolver_amd import workloads as W
 
     torch.cuda.set_device(0)
-    if a.copy_calib_mib > 0:
-        src = torch.empty(a.copy_calib_mib << 20, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+    copy_calibration(a.copy_calib_mib)
     K, Wm = a.steps, a.warmup
     KT = max(1, K // 4)  # sweeps of the separate per-launch timing pass
     wl = W.build("xy_ladder_scan", nframes=K + Wm + KT, nbins=16)
@@ -261,6 +268,7 @@ def other_workload(a):
     from mcsolver_amd import workloads as W
 
     torch.cuda.set_device(0)
+    copy_calibration(a.copy_calib_mib)
     K, Wm = a.steps, a.warmup
     KT = max(1, K // 4)
     meas = L.MEAS_PAIRS | (L.MEAS_Q if a.workload == "heis_triangular_dmi128" else 0)

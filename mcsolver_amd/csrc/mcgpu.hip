@@ -1037,6 +1037,14 @@ struct SweepArgs {
   int32_t nslots;
 };
 
+// The schedules of the full recompute and of tempering follow the global
+// sweep index (Philox counter), not the index within a call, so that a run
+// split into calls (frames, checkpoints) does the same thing at the same
+// sweeps as one call.
+__host__ __device__ inline bool recompute_due(const mcg_batch& B, int64_t global_sweep) {
+  return B.recompute_every > 0 && ((global_sweep + 1) % B.recompute_every) == 0;
+}
+
 // LDS of the tempering state a persistent launch keeps in every workgroup:
 // who and accepted-swap counts per row, rung per replica, beta and h per row
 // (and the workgroup's ladder list: at most one ladder per replica, so at
@@ -1568,7 +1576,7 @@ __device__ __forceinline__ void lds_sweeps(const mcg_graph& G, const mcg_batch& 
     dM[2] += (double)fM[2];
     if (split) continue;  // one sweep per launch; the last part to finish runs the tail
     const bool therm_end = (k + 1 == a.nthermal);
-    const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
+    const bool rec = recompute_due(B, a.philox_base + k);
     if (meas || therm_end || rec || s == a.nsweeps - 1) {
       lds_sweep_tail<NC, ST, FULL>(sp, G, B, coup, clsdat, s_red, st, C, g, lt, TPR, row, h, k, a, meas, therm_end,
                                    rec, dE, dM, key0, key1, mprev, true, pl);
@@ -1765,8 +1773,8 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
       const float bg = s_beta[rowg], hg = s_h[rowg];
       const bool meas = k >= a.nthermal && ((k - a.nthermal + 1) % a.spf) == 0;
       const bool therm_end = (k + 1 == a.nthermal);
-      const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
-      const bool pt_now = a.pt_every > 0 && ((k + 1) % a.pt_every) == 0;
+      const bool rec = recompute_due(B, a.philox_base + k);
+      const bool pt_now = a.pt_every > 0 && ((a.philox_base + k + 1) % a.pt_every) == 0;
       const int rowg_l = row0_l + s_rung[rep_l];
       const int64_t fcur = (k - a.nthermal + 1) / a.spf - 1 + a.frame0;
       if (pl.rem > 0) {  // the pool's parameters of this sweep (rows move with tempering)
@@ -1982,7 +1990,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   const int64_t k = a.k_begin;
   const bool meas = k >= a.nthermal && ((k - a.nthermal + 1) % a.spf) == 0;
   const bool therm_end = (k + 1 == a.nthermal);
-  const bool rec = B.recompute_every > 0 && ((k + 1) % B.recompute_every) == 0;
+  const bool rec = recompute_due(B, a.philox_base + k);
   const bool need_all = rec || (FULL && meas && (B.meas & MEAS_SPIN_OBS));
   uint32_t* ctr = a.sync + (size_t)w.slot * SYNC_STRIDE;
   double* xp = a.xpart + (size_t)w.slot * MAX_PARTS * 4;
@@ -2984,7 +2992,7 @@ int stream_sweeps(const mcg_batch* B, const mcg_plan* P, RunCtx& X, int64_t k0, 
     }
     const bool meas = k >= nthermal && ((k - nthermal + 1) % spf) == 0;
     const bool therm_end = (k + 1 == nthermal);
-    const bool rec = B->recompute_every > 0 && ((k + 1) % B->recompute_every) == 0;
+    const bool rec = recompute_due(*B, P->sweep0 + k);
     int mode = (rec ? 1 : 0) | (therm_end ? 2 : 0) | (meas ? 4 : 0);
     if (mode) {
       const int64_t frame = meas ? (k - nthermal + 1) / spf - 1 + P->frame0 : -1;
@@ -3219,7 +3227,7 @@ int exec_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s, RunCtx& X) 
         const int64_t frame = meas ? (k - nth + 1) / spf - 1 + P->frame0 : -1;
         if (launch_measure<NC, ST>(B, X.all_reps, nall, mode, frame, s)) return -1;
       }
-      if (pt_on && ((k + 1) % P->pt_every) == 0) {
+      if (pt_on && ((P->sweep0 + k + 1) % P->pt_every) == 0) {
         if (launch_measure<NC, ST>(B, X.all_reps, nall, 1 | 8, -1, s)) return -1;
         hipLaunchKernelGGL(k_pt, dim3((unsigned)X.hl_row0.size()), dim3(64), 0, s, *B, X.lad_row0, X.lad_n,
                            (int)X.hl_row0.size(), P->sweep0 + k);
@@ -3237,7 +3245,9 @@ int exec_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s, RunCtx& X) 
   const int64_t chunk = X.persist ? 4096 : X.nslots > 0 ? 1 : P->pt_every > 0 ? P->pt_every : total;
   int64_t bar_base = 0;
   for (int64_t k = 0; k < total;) {
-    const int64_t ns = std::min<int64_t>(chunk, total - k);
+    int64_t ns = std::min<int64_t>(chunk, total - k);
+    if (!X.persist && X.nslots == 0 && P->pt_every > 0)  // launches end where swaps are due
+      ns = std::min<int64_t>(ns, P->pt_every - (P->sweep0 + k) % P->pt_every);
     SweepArgs a{k, P->sweep0, (int32_t)ns, (int32_t)nth, spf, P->frame0, X.sync, X.xpart, X.halo};
     if (X.persist) {
       a.bar = X.bar;
@@ -3256,7 +3266,7 @@ int exec_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s, RunCtx& X) 
       a.max_export = std::max(X.max_export, 1);
       a.nslots = std::max(X.nslots, 1);
       for (int64_t q = k; q < k + ns; ++q)
-        if (X.nslots > 0 || (pt_on && ((q + 1) % P->pt_every) == 0)) ++bar_base;
+        if (X.nslots > 0 || (pt_on && ((P->sweep0 + q + 1) % P->pt_every) == 0)) ++bar_base;
       if (bar_base * (int64_t)std::max<size_t>(X.hitems.size(), BAR_GROUPS) >= ((int64_t)1 << 31))
         return fail("too many barriers in one call (%lld); split the call", (long long)bar_base);
     }
@@ -3267,7 +3277,7 @@ int exec_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s, RunCtx& X) 
     }
     if (stream_sweeps<NC, ST>(B, P, X, k, ns, nth, spf, plain, s, T)) return -1;
     k += ns;
-    if (!X.persist && pt_on && (k % P->pt_every) == 0) {
+    if (!X.persist && pt_on && ((P->sweep0 + k) % P->pt_every) == 0) {
       hipLaunchKernelGGL(k_pt, dim3((unsigned)X.hl_row0.size()), dim3(64), 0, s, *B, X.lad_row0, X.lad_n,
                          (int)X.hl_row0.size(), P->sweep0 + k - 1);
       HIPCHK(hipGetLastError());

@@ -5,6 +5,7 @@ this module, and only to verify results or to time the CPU baseline.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -55,6 +56,16 @@ def _a(x, dt):
 
 def ptr(a: np.ndarray):
     return a.ctypes.data_as(P)
+
+
+@contextlib.contextmanager
+def rng(mode: str):
+    """Integer draws from `mode` inside the block, glibc rand() after it."""
+    rng_mode(mode)
+    try:
+        yield
+    finally:
+        rng_mode("glibc")
 
 
 def rng_mode(mode: str) -> None:

@@ -116,14 +116,14 @@ def one(job):
     section, key, T, h, spec, seed, rng = job
     from mcsolver_amd import lattice as lat
     from oracle import mcref
-    mcref.rng_mode(rng)
     nth, nsw, _ = SECTIONS[section]
     Lt = lattice(spec)
-    if spec[0] == "square_ising":
-        out, _ = mcref.ising_run(lat.ising_args(Lt, T, h, nth, nsw), seed=seed)
-        return [float(v) for v in out]
-    nc = 2 if spec[0] == "square_xy" else 3
-    out, _, g = mcref.on_run(nc, lat.on_args(Lt, T, h, nth, nsw), seed=seed)
+    with mcref.rng(rng):
+        if spec[0] == "square_ising":
+            out, _ = mcref.ising_run(lat.ising_args(Lt, T, h, nth, nsw), seed=seed)
+            return [float(v) for v in out]
+        nc = 2 if spec[0] == "square_xy" else 3
+        out, _, g = mcref.on_run(nc, lat.on_args(Lt, T, h, nth, nsw), seed=seed)
     return [float(out[8]), float(out[9]), float(out[22]), float(out[26]), float(g[3]), float(g[5])]
 
 

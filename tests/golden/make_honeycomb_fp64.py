@@ -48,10 +48,10 @@ def one(job):
     T, seed, rng = job
     from mcsolver_amd import lattice as lat
     from oracle import mcref
-    mcref.rng_mode(rng)
     H = lattice()
     a = lat.on_args(H, T, 0.0, NTHERMAL, NSWEEP)
-    out, _, g = mcref.on_run(3, a, seed=seed)
+    with mcref.rng(rng):
+        out, _, g = mcref.on_run(3, a, seed=seed)
     return [float(out[8]), float(out[9]), float(out[22]), float(g[3]), float(g[5])]
 
 

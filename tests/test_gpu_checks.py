@@ -128,20 +128,23 @@ def _snap(b):
             b.state_data(), b.error_flag())
 
 
+@pytest.mark.parametrize("pt_every", [1, 3])
 @pytest.mark.parametrize("kernel", [L.KERNEL_LDS, L.KERNEL_STREAM])
-def test_checkpoint_resume_is_bit_identical(gpu, kernel, tmp_path):
+def test_checkpoint_resume_is_bit_identical(gpu, kernel, pt_every, tmp_path):
     """Tempering ladders, frames every sweep, FP64 recompute every 7 sweeps:
     10 thermal + 40 measured sweeps in one run, against 10 + 15, checkpoint
-    to a file, a new batch restored from it, then 25 more."""
+    to a file, a new batch restored from it, then 25 more.  Recomputes and
+    swaps follow the global sweep index, so the split at sweep 25 (not a
+    multiple of 7 or 3) changes nothing."""
     a = _ladders()
     a.init()
-    a.run(10, 40, 1, pt_every=1, kernel=kernel, sweep0=0)
+    a.run(10, 40, 1, pt_every=pt_every, kernel=kernel, sweep0=0)
     ref = _snap(a)
     a.close()
     del a
     b = _ladders()
     b.init()
-    b.run(10, 15, 1, pt_every=1, kernel=kernel, sweep0=0)
+    b.run(10, 15, 1, pt_every=pt_every, kernel=kernel, sweep0=0)
     f = os.path.join(str(tmp_path), "ckpt.npz")
     np.savez(f, **b.checkpoint())
     b.close()
@@ -150,7 +153,7 @@ def test_checkpoint_resume_is_bit_identical(gpu, kernel, tmp_path):
     c = _ladders()
     c.restore(dict(np.load(f)))
     assert c._next_sweep == 25
-    c.run(0, 25, 1, pt_every=1, kernel=kernel, frame0=15)
+    c.run(0, 25, 1, pt_every=pt_every, kernel=kernel, frame0=15)
     got = _snap(c)
     assert ref[-1] == 0 and got[-1] == 0
     assert np.array_equal(ref[0].view(np.uint32), got[0].view(np.uint32))

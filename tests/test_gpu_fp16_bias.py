@@ -4,7 +4,11 @@ E, C, |M|, chi and U4 at four temperatures of the honeycomb ladder, in units
 of the combined standard error of R independent runs per side.
 
 The FP64 side is tests/golden/honeycomb_fp64.json (oracle restatement of
-heisenberglib.MCMainFunction, written by tests/golden/make_honeycomb_fp64.py).
+heisenberglib.MCMainFunction with integer draws from xoshiro256**, 32 runs per
+temperature, written by tests/golden/make_honeycomb_fp64.py; with glibc
+rand(), as the reference draws, its C at T = 1.005 sits 1.5 standard errors
+higher).  The FP32 bias must also pass a chi-square over its 20
+comparisons; FP16 is reported.
 The GPU side runs the drop-in heisenberglib path (mcsolver_amd.dropin.on_main)
 with the same arguments.  Observables per run, from the reference's tuple
 (tuple[8] E, tuple[9] E^2 in beta units, tuple[22] |M| per cell) and the
@@ -19,6 +23,8 @@ import os
 
 import numpy as np
 import pytest
+
+from stats import chi2_summary, welch_z
 
 pytestmark = [pytest.mark.gpu, pytest.mark.slow]
 
@@ -42,6 +48,7 @@ def test_fp32_fp16_bias_vs_fp64_oracle(gpu):
     assert H.n == d["n"]
     R = d["runs_per_T"]
     rows, bad = [], []
+    zs = {"f32": [], "f16": []}
     for i, T in enumerate(d["temps"]):
         ref = _obs(d["runs"]["%.6f" % T], T, nlat)
         a = lat.on_args(H, T, 0.0, d["nthermal"], d["nsweep"])
@@ -57,9 +64,16 @@ def test_fp32_fp16_bias_vs_fp64_oracle(gpu):
                 mg, mr = got[:, k].mean(), ref[:, k].mean()
                 se = np.sqrt(got[:, k].var(ddof=1) / R + ref[:, k].var(ddof=1) / R)
                 z = (mg - mr) / se if se > 0 else 0.0
+                zs[store].append(welch_z(got[:, k], ref[:, k]))
                 rows.append("T=%.3f %-3s %-3s gpu %.6g  fp64 %.6g  bias %+.3g (%+.2f sigma)" %
                             (T, store, nm, mg, mr, mg - mr, z))
                 if abs(mg - mr) > 4 * se + floor * abs(mr):
                     bad.append(rows[-1])
     print("\n".join(rows), flush=True)
+    for store in ("f32", "f16"):
+        chi2, dof, lim, f2 = chi2_summary(zs[store])
+        print("%s: chi2 = %.1f for %d comparisons (99.9%% quantile %.1f); %.0f%% within 2 sigma" %
+              (store, chi2, dof, lim, 100 * f2))
+        if store == "f32":
+            assert chi2 < lim, "FP32 storage: chi2 %.1f >= %.1f" % (chi2, lim)
     assert not bad, "\n".join(bad)

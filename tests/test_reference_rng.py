@@ -38,10 +38,9 @@ def test_xoshiro256ss_known_answers():
 def test_rng_mode_switch_keeps_glibc_replay():
     """Switching to xoshiro and back leaves the glibc stream the replay
     fixtures use (mode 0 is rand() itself)."""
-    mcref.rng_mode("xoshiro")
-    mcref.lib().mcref_srand(7)
-    a = [mcref.lib().mcref_rand() for _ in range(3)]
-    mcref.rng_mode("glibc")
+    with mcref.rng("xoshiro"):
+        mcref.lib().mcref_srand(7)
+        a = [mcref.lib().mcref_rand() for _ in range(3)]
     mcref.lib().mcref_srand(7)
     b = [mcref.lib().mcref_rand() for _ in range(3)]
     assert a != b and all(0 <= v < 2 ** 31 for v in a + b)
