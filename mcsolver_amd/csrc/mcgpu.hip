@@ -435,6 +435,34 @@ __device__ __forceinline__ void metro_group_pat(const Spins<ST>& sp, int b, int 
   }
 }
 
+// same, with the spins in HBM (streaming pass): pattern ids read from HBM,
+// pattern rows (raw position deltas | coupling << 24, then the class) in LDS
+template <int NC, typename ST, bool UNI>
+__device__ __forceinline__ void metro_group_gpat(const GSpins<ST>& sp, int b, int cnt, int q, int colour,
+                                                 const GAS uint8_t* __restrict__ pat, const int32_t* pattab, int zmax,
+                                                 const GAS float* coup, const GAS float* clsdat, float beta, float h,
+                                                 uint64_t sweep, uint32_t key0, uint32_t key1, float& dEacc,
+                                                 float* dMacc, bool aniso) {
+  constexpr int G = SitesPerDraw<NC>::G;
+  const u32x4 x = philox4x32_10(
+      u32x4{(uint32_t)q, (uint32_t)sweep, (uint32_t)(sweep >> 32), TAG_METRO | ((uint32_t)colour << 8)}, key0, key1);
+  const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int lp = q * G + g;
+    if (lp < cnt) {
+      const int p = b + lp;
+      const int32_t* row = pattab + __umul24((uint32_t)pat[p], (uint32_t)pat_stride(zmax));
+      const PatLinks lk{row, p};
+      const uint32_t wa = NC == 3 ? w[0] : 0u;
+      const uint32_t wb = NC == 3 ? w[1] : NC == 2 ? w[2 * g] : 0u;
+      const uint32_t wc = NC == 3 ? w[2] : NC == 2 ? w[2 * g + 1] : w[g];
+      metro_core<NC, GSpins<ST>, PatLinks, const GAS float*, UNI>(sp, p, lk, zmax, coup, row[zmax], clsdat, beta, h,
+                                                                  wa, wb, wc, dEacc, dMacc, aniso);
+    }
+  }
+}
+
 // spin vector of site i (Ising: sigma*S)
 template <int NC, typename SV>
 __device__ __forceinline__ void spin_vec(const SV& sp, int i, const int32_t* __restrict__ cls, const float* clsdat,
@@ -620,16 +648,24 @@ __device__ __forceinline__ void majority(const SV& sp, const mcg_graph& G, const
 }
 
 // solid angle of a spin triangle, heisenbergLib.c:114-127 (atan, +-pi branch)
-__device__ __forceinline__ float signed_area(const float* a, const float* b, const float* c, float la, float lb,
-                                             float lc) {
-  const float ab = (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) / la / lb;
-  const float bc = (b[0] * c[0] + b[1] * c[1] + b[2] * c[2]) / lb / lc;
-  const float ca = (c[0] * a[0] + c[1] * a[1] + c[2] * a[2]) / lc / la;
+// (ia, ib, ic: 1/|S| of the three sites).  An observable, compared within
+// statistics: products with reciprocal lengths and one fast division stand
+// in for the reference's seven divisions.
+__device__ __forceinline__ float signed_area(const float* a, const float* b, const float* c, float ia, float ib,
+                                             float ic) {
+  const float ab = (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) * (ia * ib);
+  const float bc = (b[0] * c[0] + b[1] * c[1] + b[2] * c[2]) * (ib * ic);
+  const float ca = (c[0] * a[0] + c[1] * a[1] + c[2] * a[2]) * (ic * ia);
   const float x0 = b[1] * c[2] - b[2] * c[1], x1 = b[2] * c[0] - b[0] * c[2], x2 = b[0] * c[1] - b[1] * c[0];
   const float re = 1.0f + ab + bc + ca;
-  const float im = (a[0] * x0 + a[1] * x1 + a[2] * x2) / la / lb / lc;
+  const float im = (a[0] * x0 + a[1] * x1 + a[2] * x2) * (ia * ib * ic);
   if (fabsf(re) < 1e-6f) return im > 0.0f ? (float)REF_PI : -(float)REF_PI;
-  return 2.0f * atanf(im / re);
+  return 2.0f * atanf(__fdividef(im, re));
+}
+
+// 1/|S| of a site class: sqrt of the stored 1/S^2
+__device__ __forceinline__ float inv_len(const float* clsdat, const int32_t* cls, int i) {
+  return sqrtf(clsdat[cls[i] * 8 + 4]);
 }
 
 // Measures one frame.  Tracked E and M (FP64) of the replica are passed in
@@ -783,19 +819,31 @@ __device__ __forceinline__ void frame_terms(const mcg_graph& G, const mcg_batch&
 // up to date at once (thread MCG_A_MM; mprev, if have_prev, is M' read ahead
 // by that thread), and the rest is applied to the accumulators now or, with
 // defer, left staged for the caller's flush_frame.
+// [lo, hi): share `part` of `np` of a loop over cnt items
+__device__ __forceinline__ void slice(int cnt, int part, int np, int& lo, int& hi) {
+  lo = (int)((int64_t)cnt * part / np);
+  hi = (int)((int64_t)cnt * (part + 1) / np);
+}
+
+// The spin-derived sums of one frame over share `part` of `np` of every
+// loop (pairs, circuits, block spins, groups), left in F (valid in the
+// group's first thread): F[0..8) pairs, F[8] solid angles, F[16..29) block
+// spins, F[32..) groups.  np = 1: the whole frame.
+static constexpr int FSUMS = 88;  // frame sums per replica (F[0, 88))
 template <int NC, typename SV, bool FULL = true>
-__device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& B, const double* st, int row,
-                              int64_t frame, uint32_t key0, uint32_t key1, float h, double mprev = 0.0,
-                              bool have_prev = false, bool defer = false) {
+__device__ void frame_sums(const SV& sp, const FrameCtx& C, const mcg_batch& B, int64_t frame, uint32_t key0,
+                           uint32_t key1, float h, int part, int np) {
   const mcg_graph& G = *C.G;
   const int lt = C.lt, TPR = C.TPR;
   double* F = C.s_frm;  // leader-only use
   double v[RK];
   const int flags = FULL ? B.meas : (B.meas & ~MEAS_SPIN_OBS);
+  int lo, hi;
   // --- correlation pairs: S_i, S_j, sum s_i.s_j, last-site spin
   if ((flags & MCG_MEAS_PAIRS) && G.nlat > 0) {
     for (int k = 0; k < RK; ++k) v[k] = 0.0;
-    for (int p = lt; p < G.nlat; p += TPR) {
+    slice(G.nlat, part, np, lo, hi);
+    for (int p = lo + lt; p < hi; p += TPR) {
       float a[3], b[3];
       spin_vec<NC>(sp, G.pairs[2 * p], G.cls, C.clsdat, a);
       spin_vec<NC>(sp, G.pairs[2 * p + 1], G.cls, C.clsdat, b);
@@ -803,7 +851,7 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
       v[3] += b[0]; v[4] += b[1]; v[5] += b[2];
       v[6] += (double)(a[0] * b[0] + a[1] * b[1] + a[2] * b[2]);
     }
-    if (lt == 0) {
+    if (lt == 0 && part == 0) {
       float t[3];
       spin_vec<NC>(sp, G.pos[G.n - 1], G.cls, C.clsdat, t);
       v[7] = t[0];
@@ -815,14 +863,22 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
   // --- topological charge
   if ((flags & MCG_MEAS_Q) && NC == 3 && G.ncirc > 0) {
     for (int k = 0; k < RK; ++k) v[k] = 0.0;
-    for (int p = lt; p < G.ncirc; p += TPR) {
+    slice(G.ncirc, part, np, lo, hi);
+    // one spin length for every class: no per-site class lookups
+    float il = sqrtf(C.clsdat[4]);
+    bool one_len = G.ncls <= 64;
+    for (int q = 1; q < G.ncls && one_len; ++q) one_len = C.clsdat[q * 8 + 4] == C.clsdat[4];
+    for (int p = lo + lt; p < hi; p += TPR) {
       const int ia = G.circ[3 * p], ib = G.circ[3 * p + 1], ic = G.circ[3 * p + 2];
       float a[3], b[3], c[3];
       spin_vec<NC>(sp, ia, G.cls, C.clsdat, a);
       spin_vec<NC>(sp, ib, G.cls, C.clsdat, b);
       spin_vec<NC>(sp, ic, G.cls, C.clsdat, c);
-      v[0] += (double)signed_area(a, b, c, fabsf(C.clsdat[G.cls[ia] * 8]), fabsf(C.clsdat[G.cls[ib] * 8]),
-                                  fabsf(C.clsdat[G.cls[ic] * 8]));
+      if (one_len)
+        v[0] += (double)signed_area(a, b, c, il, il, il);
+      else
+        v[0] += (double)signed_area(a, b, c, inv_len(C.clsdat, G.cls, ia), inv_len(C.clsdat, G.cls, ib),
+                                    inv_len(C.clsdat, G.cls, ic));
     }
     group_sum<1>(v, C.s_red, C.g, TPR);
     if (lt == 0) F[8] = v[0];
@@ -832,7 +888,8 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
     for (int k = 0; k < RK; ++k) v[k] = 0.0;
     double w[RK];
     for (int k = 0; k < RK; ++k) w[k] = 0.0;
-    for (int p = lt; p < G.nlat; p += TPR) {
+    slice(G.nlat, part, np, lo, hi);
+    for (int p = lo + lt; p < hi; p += TPR) {
       const int ia = G.pairs[2 * p], ib = G.pairs[2 * p + 1];
       const bool ca = G.rrank[ia] >= 0, cb = G.rrank[ib] >= 0;
       float ma[3] = {0, 0, 0}, mb[3] = {0, 0, 0};
@@ -851,7 +908,8 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
         w[2] += 1.0;
       }
     }
-    for (int ri = lt; ri < G.nr; ri += TPR) {
+    slice(G.nr, part, np, lo, hi);
+    for (int ri = lo + lt; ri < hi; ri += TPR) {
       const int j = G.rorb[ri];
       float m[3];
       majority<NC>(sp, G, C.clsdat, j, frame, key0, key1, m);
@@ -899,8 +957,9 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
   const int ng = (flags & MCG_MEAS_GROUPS) ? G.ngroup : 0;
   for (int g0 = 0; g0 < ng; g0 += 2) {
     for (int k = 0; k < RK; ++k) v[k] = 0.0;
+    slice(G.gmax, part, np, lo, hi);
     for (int q = 0; q < 2 && g0 + q < ng; ++q) {
-      for (int k = lt; k < G.gmax; k += TPR) {
+      for (int k = lo + lt; k < hi; k += TPR) {
         const int o = G.groups[(g0 + q) * G.gmax + k];
         if (o < 0) continue;
         float s[3];
@@ -912,7 +971,30 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
     if (lt == 0)
       for (int k = 0; k < 6; ++k) F[32 + 3 * g0 + k] = v[k];
   }
+}
 
+// One measured frame of the group's replica at `row`: the spin-derived sums
+// (frame_sums), then frame_apply.
+template <int NC>
+__device__ void frame_apply(const FrameCtx& C, const mcg_batch& B, const double* st, int row, int64_t frame,
+                            int flags, double mprev, bool have_prev, bool defer);
+template <int NC, typename SV, bool FULL = true>
+__device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& B, const double* st, int row,
+                              int64_t frame, uint32_t key0, uint32_t key1, float h, double mprev = 0.0,
+                              bool have_prev = false, bool defer = false) {
+  frame_sums<NC, SV, FULL>(sp, C, B, frame, key0, key1, h, 0, 1);
+  frame_apply<NC>(C, B, st, row, frame, FULL ? B.meas : (B.meas & ~MEAS_SPIN_OBS), mprev, have_prev, defer);
+}
+
+// This frame's contribution to every FP64 accumulator of the row, from the
+// tracked state st and the spin-derived sums in C.s_frm (valid in the
+// group's first thread); called by every thread of the group.
+template <int NC>
+__device__ void frame_apply(const FrameCtx& C, const mcg_batch& B, const double* st, int row, int64_t frame,
+                            int flags, double mprev, bool have_prev, bool defer) {
+  const mcg_graph& G = *C.G;
+  const int lt = C.lt;
+  double* F = C.s_frm;
   // --- leader: this frame's contribution to every FP64 accumulator of the
   //     row (isingLib.c:388-431, heisenbergLib.c:677-830 / xyLib.c:605-757),
   //     staged in LDS; the group's threads then apply them to global memory
@@ -942,10 +1024,12 @@ __device__ void measure_frame(const SV& sp, const FrameCtx& C, const mcg_batch& 
 static constexpr int NREC = 5;
 template <int NC, typename SV>
 __device__ void recompute(const SV& sp, const mcg_graph& G, const float* coup, const float* clsdat, float h,
-                          double* s_red, int g, int lt, int TPR, double* out) {
+                          double* s_red, int g, int lt, int TPR, double* out, int part = 0, int np = 1) {
   double v[RK];
   for (int k = 0; k < RK; ++k) v[k] = 0.0;
-  for (int i = lt; i < G.n; i += TPR) {
+  int lo, hi;
+  slice(G.n, part, np, lo, hi);
+  for (int i = lo + lt; i < hi; i += TPR) {
     v[0] += (double)site_energy<NC>(sp, i, G.link, G.lcid, G.zmax, coup, G.cls, clsdat, h);
     float s[3];
     spin_vec<NC>(sp, i, G.cls, clsdat, s);
@@ -1035,6 +1119,8 @@ struct SweepArgs {
   double* mprev3;            // [3][nrow] M' of each row's last frames (frame f in slot f % 3)
   int32_t max_export;
   int32_t nslots;
+  int32_t rec_host;          // the full recompute is the host's (a launch ends at each
+                             // recompute sweep, then k_measure recomputes every replica)
 };
 
 // The schedules of the full recompute and of tempering follow the global
@@ -1773,7 +1859,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
       const float bg = s_beta[rowg], hg = s_h[rowg];
       const bool meas = k >= a.nthermal && ((k - a.nthermal + 1) % a.spf) == 0;
       const bool therm_end = (k + 1 == a.nthermal);
-      const bool rec = recompute_due(B, a.philox_base + k);
+      const bool rec = !a.rec_host && recompute_due(B, a.philox_base + k);
       const bool pt_now = a.pt_every > 0 && ((a.philox_base + k + 1) % a.pt_every) == 0;
       const int rowg_l = row0_l + s_rung[rep_l];
       const int64_t fcur = (k - a.nthermal + 1) / a.spf - 1 + a.frame0;
@@ -2028,54 +2114,88 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
 }
 
 // ---------------------------------------------------------------------------
-// k_pass: one colour pass, spins in global memory.  grid = (chunks, nrep)
+// k_pass: one colour class of every listed replica, spins in HBM / L2.
+// grid = (chunks, nrep) of 256 threads; thread t of chunk x takes the Philox
+// groups (x * PASS_GPT + u) * 256 + t of the colour, u < PASS_GPT, so that
+// a wave touches consecutive positions and each thread's reduction is shared
+// by PASS_GPT groups.  Graphs with link patterns read their neighbours as
+// position deltas from pattern rows staged in LDS (no link-table traffic and
+// no load that waits on another); the others read the global link table.
+// Changes of E and M: FP32 per thread, FP64 DPP wave sums, one device
+// atomic per value per block.
 // ---------------------------------------------------------------------------
+// Philox groups per thread and the unrolling of that loop (measured on the
+// cubic 64^3 and triangular 128^2 streaming sets: 8 groups, not unrolled,
+// is 7 % faster than 4 unrolled and 22 % faster than 2)
+#ifndef MCG_PASS_GPT
+#define MCG_PASS_GPT 8
+#endif
+#ifndef MCG_PASS_UNROLL
+#define MCG_PASS_UNROLL 1
+#endif
+static constexpr int PASS_GPT = MCG_PASS_GPT;
+static constexpr int PASS_PAT_INTS = 4096;  // LDS for pattern rows (16 KB)
 template <int NC, typename ST>
 __global__ __launch_bounds__(256) void k_pass(mcg_batch B, const int32_t* __restrict__ rep_list, int colour,
                                               int64_t sweep) {
+  __shared__ int32_t s_pat[PASS_PAT_INTS];
+  __shared__ double s_red[4 * 4];
   const int rep = rep_list[blockIdx.y];
   const mcg_replica rp = B.reps[rep];
   const mcg_graph G = B.graphs[rp.graph];
+  const int zmax = G.zmax;
+  const bool pats = G.npat > 0 && G.lcid == nullptr && G.npat * pat_stride(zmax) <= PASS_PAT_INTS;
+  int anz = 0;
+  if (pats) {
+    for (int t = threadIdx.x; t < G.npat * (zmax + 1); t += blockDim.x)
+      s_pat[(t / (zmax + 1)) * pat_stride(zmax) + t % (zmax + 1)] = G.pattab[t];
+    // does any site class carry an on-site term? (metro_core skips them if not)
+    for (int t = threadIdx.x; t < G.ncls; t += blockDim.x)
+      anz |= (G.clsdat[t * 8 + 1] != 0.0f) | (G.clsdat[t * 8 + 2] != 0.0f) | (G.clsdat[t * 8 + 3] != 0.0f);
+  }
+  const bool aniso = __syncthreads_or(anz) != 0 || !pats;
   float dE = 0.0f, dM[3] = {0.0f, 0.0f, 0.0f};
   if (colour < G.ncol) {
     const int b = G.col_ptr[colour], cnt = G.col_ptr[colour + 1] - b;
     const int ngrp = (cnt + SitesPerDraw<NC>::G - 1) / SitesPerDraw<NC>::G;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < ngrp) {
-      const int row = rp.row0 + B.rung[rep];
-      const GSpins<ST> sp{gas((ST*)B.spins + rp.spin_off), G.n};
-      const GAS int32_t* lcid = G.lcid ? gas(G.lcid) : nullptr;
-      if (G.ucid >= 0)
-        metro_group<NC, true>(sp, b, cnt, q, colour, gas(G.link), lcid, G.zmax, gas(G.coup) + G.ucid * 9, gas(G.cls),
-                              gas(G.clsdat), B.row_beta[row], B.row_h[row], (uint64_t)sweep, B.seed_lo,
-                              B.seed_hi ^ rp.serial, dE, dM);
-      else
-        metro_group<NC>(sp, b, cnt, q, colour, gas(G.link), lcid, G.zmax, gas(G.coup), gas(G.cls), gas(G.clsdat),
-                        B.row_beta[row], B.row_h[row], (uint64_t)sweep, B.seed_lo, B.seed_hi ^ rp.serial, dE, dM);
+    const int row = rp.row0 + B.rung[rep];
+    const float beta = B.row_beta[row], h = B.row_h[row];
+    const uint32_t key1 = B.seed_hi ^ rp.serial;
+    const GSpins<ST> sp{gas((ST*)B.spins + rp.spin_off), G.n};
+    const GAS int32_t* lcid = G.lcid ? gas(G.lcid) : nullptr;
+#pragma unroll MCG_PASS_UNROLL
+    for (int u = 0; u < PASS_GPT; ++u) {
+      const int q = (blockIdx.x * PASS_GPT + u) * (int)blockDim.x + (int)threadIdx.x;
+      if (q >= ngrp) break;
+      if (pats) {
+        if (G.ucid >= 0)
+          metro_group_gpat<NC, ST, true>(sp, b, cnt, q, colour, gas(G.pat), s_pat, zmax, gas(G.coup) + G.ucid * 9,
+                                         gas(G.clsdat), beta, h, (uint64_t)sweep, B.seed_lo, key1, dE, dM, aniso);
+        else
+          metro_group_gpat<NC, ST, false>(sp, b, cnt, q, colour, gas(G.pat), s_pat, zmax, gas(G.coup),
+                                          gas(G.clsdat), beta, h, (uint64_t)sweep, B.seed_lo, key1, dE, dM, aniso);
+      } else if (G.ucid >= 0) {
+        metro_group<NC, true>(sp, b, cnt, q, colour, gas(G.link), lcid, zmax, gas(G.coup) + G.ucid * 9, gas(G.cls),
+                              gas(G.clsdat), beta, h, (uint64_t)sweep, B.seed_lo, key1, dE, dM);
+      } else {
+        metro_group<NC>(sp, b, cnt, q, colour, gas(G.link), lcid, zmax, gas(G.coup), gas(G.cls), gas(G.clsdat),
+                        beta, h, (uint64_t)sweep, B.seed_lo, key1, dE, dM);
+      }
     }
   }
-  // block reduction, one FP64 atomic per value per block
-  __shared__ double s_red[4 * RK];
-  double v[4] = {dE, dM[0], dM[1], dM[2]};
-  for (int k = 0; k < 4; ++k) {
-    double x = v[k];
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    v[k] = x;
-  }
-  if ((threadIdx.x & 63) == 0)
-    for (int k = 0; k < 4; ++k) s_red[(threadIdx.x >> 6) * 4 + k] = v[k];
+  // every lane reaches here: DPP sums per wave, the waves' totals through LDS
+  double v[4] = {(double)dE, (double)dM[0], (double)dM[1], (double)dM[2]};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = wave_sum_f64(v[k]);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 63)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s_red[wave * 4 + k] = v[k];
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double t[4] = {0, 0, 0, 0};
-    for (int q = 0; q < (int)(blockDim.x >> 6); ++q)
-      for (int k = 0; k < 4; ++k) t[k] += s_red[q * 4 + k];
-    if (t[0] != 0.0 || t[1] != 0.0 || t[2] != 0.0 || t[3] != 0.0) {
-      double* st = B.state + (size_t)rep * MCG_NSTATE;
-      atomicAdd(st + MCG_S_E, t[0]);
-      atomicAdd(st + MCG_S_M, t[1]);
-      atomicAdd(st + MCG_S_M + 1, t[2]);
-      atomicAdd(st + MCG_S_M + 2, t[3]);
-    }
+  if (threadIdx.x < 4) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_red[w * 4 + threadIdx.x];
+    if (t != 0.0) atomicAdd(B.state + (size_t)rep * MCG_NSTATE + MCG_S_E + threadIdx.x, t);
   }
 }
 
@@ -2121,6 +2241,83 @@ __global__ __launch_bounds__(1024) void k_measure(mcg_batch B, const int32_t* __
   }
   __syncthreads();
   if (threadIdx.x < MCG_NSTATE) B.state[(size_t)rep * MCG_NSTATE + threadIdx.x] = st[threadIdx.x];
+}
+
+// ---------------------------------------------------------------------------
+// Split measurement (streaming and Wolff paths): k_frame_part gives np
+// workgroups to each replica, each reducing its share of the frame sums and
+// of the full recompute and adding them into the replica's row of fsum;
+// k_frame_end (one wave per replica) turns a row into the recompute
+// bookkeeping and the frame's accumulator terms, and clears it.  mode bits
+// as k_measure: 1 recompute, 2 set E_ref, 4 measure frame, 8 reset drift.
+// ---------------------------------------------------------------------------
+static constexpr int FSLOTS = FSUMS + 8;  // per replica: frame sums, then NREC recompute sums
+template <int NC, typename ST>
+__global__ __launch_bounds__(256) void k_frame_part(mcg_batch B, const int32_t* __restrict__ rep_list, int mode,
+                                                    int64_t frame, double* __restrict__ fsum) {
+  const int rep = rep_list[blockIdx.y];
+  const mcg_replica rp = B.reps[rep];
+  const mcg_graph G = B.graphs[rp.graph];
+  __shared__ double s_red[4 * RK];
+  __shared__ double s_frm[FSUMS];
+  const Spins<ST> sp{(ST*)B.spins + rp.spin_off, G.n};
+  const int row = rp.row0 + B.rung[rep];
+  const float h = B.row_h[row];
+  const int part = blockIdx.x, np = gridDim.x;
+  double* out = fsum + (size_t)rep * FSLOTS;
+  if (mode & 1) {
+    double r[NREC];
+    recompute<NC>(sp, G, G.coup, G.clsdat, h, s_red, 0, threadIdx.x, blockDim.x, r, part, np);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < NREC; ++k)
+        if (r[k] != 0.0) atomicAdd(out + FSUMS + k, r[k]);
+  }
+  if (mode & 4) {
+    for (int t = threadIdx.x; t < FSUMS; t += blockDim.x) s_frm[t] = 0.0;
+    __syncthreads();
+    FrameCtx C{&G, G.coup, G.clsdat, s_red, s_frm, 0, (int)threadIdx.x, (int)blockDim.x};
+    frame_sums<NC>(sp, C, B, frame, B.seed_lo, B.seed_hi ^ rp.serial, h, part, np);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < FSUMS; ++k)
+        if (s_frm[k] != 0.0) atomicAdd(out + k, s_frm[k]);
+  }
+}
+
+template <int NC, typename ST>
+__global__ __launch_bounds__(64) void k_frame_end(mcg_batch B, const int32_t* __restrict__ rep_list, int mode,
+                                                  int64_t frame, double* __restrict__ fsum) {
+  const int rep = rep_list[blockIdx.x];
+  const mcg_replica rp = B.reps[rep];
+  const mcg_graph G = B.graphs[rp.graph];
+  __shared__ double s_red[RK];
+  __shared__ double s_frm[FRM];
+  __shared__ double st[MCG_NSTATE];
+  const int lt = threadIdx.x;
+  const int row = rp.row0 + B.rung[rep];
+  double* in = fsum + (size_t)rep * FSLOTS;
+  if (lt < MCG_NSTATE) st[lt] = B.state[(size_t)rep * MCG_NSTATE + lt];
+  for (int t = lt; t < FSUMS; t += blockDim.x) s_frm[t] = in[t];
+  __syncthreads();
+  if ((mode & 1) && lt == 0) {
+    double r[NREC];
+    for (int k = 0; k < NREC; ++k) r[k] = in[FSUMS + k];
+    double d = 0.0;
+    if (!(mode & 8)) {
+      d = fabs(st[MCG_S_E] - r[0]);
+      if (d > st[MCG_S_DRIFT]) st[MCG_S_DRIFT] = d;
+    }
+    recompute_checks<ST>(B, G, r, d);
+    for (int k = 0; k < 4; ++k) st[MCG_S_E + k] = r[k];
+  }
+  if ((mode & 2) && lt == 0) st[MCG_S_EREF] = st[MCG_S_E];
+  __syncthreads();
+  if (mode & 4) {
+    FrameCtx C{&G, G.coup, G.clsdat, s_red, s_frm, 0, lt, (int)blockDim.x};
+    frame_apply<NC>(C, B, st, row, frame, B.meas, 0.0, false, false);
+  }
+  __syncthreads();
+  for (int t = lt; t < FSLOTS; t += blockDim.x) in[t] = 0.0;
+  if (lt < MCG_NSTATE) B.state[(size_t)rep * MCG_NSTATE + lt] = st[lt];
 }
 
 // ---------------------------------------------------------------------------
@@ -2459,6 +2656,8 @@ struct RunCtx {
   size_t lds_max = 0;
   double lds_updates_per_sweep = 0;
   int device = 0;
+  int stream_ncol = 0, stream_maxcnt = 0;  // colours and largest colour class of the streamed graphs
+  int frame_np_stream = 1, frame_np_all = 1;  // workgroups per replica of the split measurement
   // what the plan was made for: a run whose plan or batch differs in any of
   // these is refused
   int32_t k_kernel = 0, k_block = 0, k_rpb = 0, k_parts = 0, k_nopersist = 0, k_pt = 0;
@@ -2470,7 +2669,8 @@ struct RunCtx {
   std::vector<unsigned char> hstage;  // the tables as uploaded
   // offsets of the pieces, then their device addresses
   size_t o_items = 0, o_item_reps = 0, o_stream = 0, o_all = 0, o_lrow0 = 0, o_ln = 0, o_halo = 0, o_repnp = 0,
-         o_blklad = 0, o_sync = 0, o_xpart = 0, o_bar = 0, o_eslot = 0, o_mslot = 0, o_xbuf = 0, o_mprev3 = 0;
+         o_blklad = 0, o_sync = 0, o_xpart = 0, o_bar = 0, o_eslot = 0, o_mslot = 0, o_xbuf = 0, o_mprev3 = 0,
+         o_fsum = 0;
   const WorkItem* items = nullptr;
   const ItemRep* item_reps = nullptr;
   const int32_t *stream_reps = nullptr, *all_reps = nullptr, *lad_row0 = nullptr, *lad_n = nullptr,
@@ -2478,6 +2678,7 @@ struct RunCtx {
   const uint8_t* rep_np = nullptr;
   uint32_t *sync = nullptr, *bar = nullptr;
   double *xpart = nullptr, *eslot = nullptr, *mslot = nullptr, *mprev3 = nullptr;
+  double* fsum = nullptr;  // split measurement: FSLOTS per replica, zero between frames
   void* xbuf = nullptr;
 };
 
@@ -2700,7 +2901,7 @@ int plan_lds(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool force_stream
   // automatic splits only where the launch can be persistent: otherwise a
   // launch with split replicas runs one sweep and restages them every sweep,
   // which costs more than the imbalance it removes
-  const bool auto_split = !(B->meas & MEAS_SPIN_OBS) && B->recompute_every <= 0 && !P->no_persist;
+  const bool auto_split = !(B->meas & MEAS_SPIN_OBS) && !P->no_persist;
   struct Split {
     int gi, rep, np;
   };
@@ -2883,7 +3084,7 @@ int launch_lds(const mcg_batch* B, RunCtx& X, const SweepArgs& a, hipStream_t s,
 // Can this run use a persistent launch?  All replicas on the LDS path, one
 // link mode, no spin-derived observables, tempering or split replicas to
 // synchronise (otherwise a plain multi-sweep launch already is persistent),
-// no periodic recompute of split replicas, the tempering state fits in LDS,
+// the tempering state fits in LDS,
 // and every workgroup resident at once.
 template <int NC, typename ST>
 int check_persist(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool* ok) {
@@ -2896,7 +3097,6 @@ int check_persist(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool* ok) {
   if (B->meas & MEAS_SPIN_OBS) return 0;
   const bool pt = P->pt_every > 0 && !X.hl_row0.empty();
   if (!pt && X.nslots == 0) return 0;
-  if (X.nslots > 0 && B->recompute_every > 0) return 0;
   const size_t bytes = lds_bytes_for<NC, ST>(B, X, 0, (int)X.hitems.size()) + persist_lds_bytes(B->nrow, B->nrep);
   if (bytes > X.lds_max) return 0;
   const void* f = m == 0   ? (const void*)k_lds<NC, ST, 0, false, true>
@@ -2916,22 +3116,13 @@ int check_persist(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool* ok) {
   return 0;
 }
 
-int max_colour_count(const mcg_batch* B, const std::vector<int32_t>& reps, int* ncol_max) {
-  int mx = 0, nc = 0;
-  for (int r : reps) {
-    const mcg_graph& G = B->graphs_host[B->reps_host[r].graph];
-    mx = std::max(mx, G.n);
-    nc = std::max(nc, G.ncol);
-  }
-  *ncol_max = nc;
-  return mx;
-}
 
 template <int NC, typename ST>
-int launch_pass(const mcg_batch* B, const int32_t* dreps, int nreps, int maxn, int colour, int64_t sweep,
+int launch_pass(const mcg_batch* B, const int32_t* dreps, int nreps, int maxcnt, int colour, int64_t sweep,
                 hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   if (nreps == 0) return 0;
-  const unsigned chunks = (unsigned)((maxn + 255) / 256);
+  const int ngrp = (maxcnt + SitesPerDraw<NC>::G - 1) / SitesPerDraw<NC>::G;
+  const unsigned chunks = (unsigned)std::max(1, (ngrp + 256 * PASS_GPT - 1) / (256 * PASS_GPT));
   hipExtLaunchKernelGGL((k_pass<NC, ST>), dim3(chunks, (unsigned)nreps), dim3(256), 0, s, e0, e1, 0, *B, dreps,
                         colour, sweep);
   HIPCHK(hipGetLastError());
@@ -2942,6 +3133,28 @@ template <int NC, typename ST>
 int launch_measure(const mcg_batch* B, const int32_t* dreps, int nreps, int mode, int64_t frame, hipStream_t s) {
   if (nreps == 0) return 0;
   hipLaunchKernelGGL((k_measure<NC, ST>), dim3((unsigned)nreps), dim3(1024), 0, s, *B, dreps, mode, frame);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// workgroups per replica of the split measurement: about four per CU over
+// the batch, at least 1024 sites each
+int frame_parts(int nreps, int nmax, int ncu) {
+  if (nreps <= 0) return 1;
+  const int want = (4 * std::max(ncu, 1) + nreps - 1) / nreps;
+  return std::max(1, std::min({want, std::max(1, nmax / 1024), 65535}));
+}
+
+template <int NC, typename ST>
+int launch_frame(const mcg_batch* B, const int32_t* dreps, int nreps, int mode, int64_t frame, double* fsum, int np,
+                 hipStream_t s) {
+  if (nreps == 0) return 0;
+  if (mode & 5) {
+    hipLaunchKernelGGL((k_frame_part<NC, ST>), dim3((unsigned)np, (unsigned)nreps), dim3(256), 0, s, *B, dreps, mode,
+                       frame, fsum);
+    HIPCHK(hipGetLastError());
+  }
+  hipLaunchKernelGGL((k_frame_end<NC, ST>), dim3((unsigned)nreps), dim3(64), 0, s, *B, dreps, mode, frame, fsum);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -2966,8 +3179,8 @@ template <int NC, typename ST>
 int stream_sweeps(const mcg_batch* B, const mcg_plan* P, RunCtx& X, int64_t k0, int64_t nsw, int64_t nthermal,
                   int spf, bool plain, hipStream_t s, LaunchTimer& T) {
   if (X.hstream.empty()) return 0;
-  int ncolmax = 0;
-  const int maxn = max_colour_count(B, X.hstream, &ncolmax);
+  const int ncolmax = X.stream_ncol;
+  const int maxn = X.stream_maxcnt;  // largest colour class of the streamed graphs
   const int32_t* dreps = X.stream_reps;
   const int nrs = (int)X.hstream.size();
   for (int64_t q = 0; q < nsw; ++q) {
@@ -2999,7 +3212,7 @@ int stream_sweeps(const mcg_batch* B, const mcg_plan* P, RunCtx& X, int64_t k0, 
       if (plain) {
         for (int i = 0; i < nrs; ++i)
           if (launch_measure<NC, ST>(B, dreps + i, 1, mode, frame, s)) return -1;
-      } else if (launch_measure<NC, ST>(B, dreps, nrs, mode, frame, s)) {
+      } else if (launch_frame<NC, ST>(B, dreps, nrs, mode, frame, X.fsum, X.frame_np_stream, s)) {
         return -1;
       }
     }
@@ -3042,6 +3255,27 @@ int prepare_typed(const mcg_batch* B, const mcg_plan* P, RunCtx& X) {
       X.hl_row0.push_back(R.row0);
       X.hl_n.push_back(R.nrung);
     }
+  }
+  // colour classes of the streamed graphs (the streaming grid covers the largest)
+  X.stream_ncol = X.stream_maxcnt = 0;
+  {
+    std::vector<char> seen(B->ngraph, 0);
+    for (int r : X.hstream) {
+      const int gi = B->reps_host[r].graph;
+      if (seen[gi]) continue;
+      seen[gi] = 1;
+      const mcg_graph& G = B->graphs_host[gi];
+      std::vector<int32_t> cp(G.ncol + 1);
+      HIPCHK(hipMemcpy(cp.data(), G.col_ptr, cp.size() * 4, hipMemcpyDeviceToHost));
+      X.stream_ncol = std::max(X.stream_ncol, G.ncol);
+      for (int c = 0; c < G.ncol; ++c) X.stream_maxcnt = std::max(X.stream_maxcnt, cp[c + 1] - cp[c]);
+    }
+  }
+  {
+    int nmax = 1;
+    for (int gi = 0; gi < B->ngraph; ++gi) nmax = std::max(nmax, B->graphs_host[gi].n);
+    X.frame_np_stream = frame_parts((int)X.hstream.size(), nmax, X.ncu);
+    X.frame_np_all = frame_parts(B->nrep, nmax, X.ncu);
   }
   X.lds_updates_per_sweep = 0;
   for (const WorkItem& w : X.hitems)
@@ -3099,6 +3333,7 @@ int prepare_typed(const mcg_batch* B, const mcg_plan* P, RunCtx& X) {
   X.o_blklad = take(&off, bl.size() * 4);
   X.tab_bytes = off;
   X.reset_off = off;
+  X.o_fsum = take(&off, (size_t)B->nrep * FSLOTS * 8);
   X.o_sync = take(&off, (size_t)X.nslots * SYNC_STRIDE * 4);
   X.o_xpart = take(&off, (size_t)X.nslots * MAX_PARTS * 4 * 8);
   if (X.persist) {
@@ -3165,6 +3400,7 @@ int bind_region(RunCtx& X, unsigned char* dev, hipStream_t s) {
   X.mslot = (double*)at(X.o_mslot);
   X.xbuf = at(X.o_xbuf);
   X.mprev3 = (double*)at(X.o_mprev3);
+  X.fsum = (double*)at(X.o_fsum);
   if (X.tab_bytes && hipMemcpyAsync(dev, X.hstage.data(), X.tab_bytes, hipMemcpyHostToDevice, s) != hipSuccess)
     return fail("hipMemcpyAsync of the run tables failed");
   return 0;
@@ -3225,10 +3461,10 @@ int exec_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s, RunCtx& X) 
       const int mode = 1 | 8 | (therm_end ? 2 : 0) | (meas ? 4 : 0);
       if (meas || therm_end || k + 1 == total) {
         const int64_t frame = meas ? (k - nth + 1) / spf - 1 + P->frame0 : -1;
-        if (launch_measure<NC, ST>(B, X.all_reps, nall, mode, frame, s)) return -1;
+        if (launch_frame<NC, ST>(B, X.all_reps, nall, mode, frame, X.fsum, X.frame_np_all, s)) return -1;
       }
       if (pt_on && ((P->sweep0 + k + 1) % P->pt_every) == 0) {
-        if (launch_measure<NC, ST>(B, X.all_reps, nall, 1 | 8, -1, s)) return -1;
+        if (launch_frame<NC, ST>(B, X.all_reps, nall, 1 | 8, -1, X.fsum, X.frame_np_all, s)) return -1;
         hipLaunchKernelGGL(k_pt, dim3((unsigned)X.hl_row0.size()), dim3(64), 0, s, *B, X.lad_row0, X.lad_n,
                            (int)X.hl_row0.size(), P->sweep0 + k);
         HIPCHK(hipGetLastError());
@@ -3248,6 +3484,10 @@ int exec_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s, RunCtx& X) 
     int64_t ns = std::min<int64_t>(chunk, total - k);
     if (!X.persist && X.nslots == 0 && P->pt_every > 0)  // launches end where swaps are due
       ns = std::min<int64_t>(ns, P->pt_every - (P->sweep0 + k) % P->pt_every);
+    // split replicas in a persistent launch: launches end where a full
+    // recompute is due, and the host recomputes every replica after them
+    const bool rec_host = X.persist && X.nslots > 0 && B->recompute_every > 0;
+    if (rec_host) ns = std::min<int64_t>(ns, B->recompute_every - (P->sweep0 + k) % B->recompute_every);
     SweepArgs a{k, P->sweep0, (int32_t)ns, (int32_t)nth, spf, P->frame0, X.sync, X.xpart, X.halo};
     if (X.persist) {
       a.bar = X.bar;
@@ -3265,6 +3505,7 @@ int exec_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s, RunCtx& X) 
       a.mprev3 = X.mprev3;
       a.max_export = std::max(X.max_export, 1);
       a.nslots = std::max(X.nslots, 1);
+      a.rec_host = rec_host ? 1 : 0;
       for (int64_t q = k; q < k + ns; ++q)
         if (X.nslots > 0 || (pt_on && ((P->sweep0 + q + 1) % P->pt_every) == 0)) ++bar_base;
       if (bar_base * (int64_t)std::max<size_t>(X.hitems.size(), BAR_GROUPS) >= ((int64_t)1 << 31))
@@ -3276,6 +3517,10 @@ int exec_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s, RunCtx& X) 
       T.updates += X.lds_updates_per_sweep * (double)ns;
     }
     if (stream_sweeps<NC, ST>(B, P, X, k, ns, nth, spf, plain, s, T)) return -1;
+    if (rec_host && recompute_due(*B, P->sweep0 + k + ns - 1)) {
+      const int mode = 1 | (k + ns == nth ? 2 : 0);  // recompute (+ energy reference at thermalisation end)
+      if (launch_frame<NC, ST>(B, X.all_reps, nall, mode, -1, X.fsum, X.frame_np_all, s)) return -1;
+    }
     k += ns;
     if (!X.persist && pt_on && ((P->sweep0 + k) % P->pt_every) == 0) {
       hipLaunchKernelGGL(k_pt, dim3((unsigned)X.hl_row0.size()), dim3(64), 0, s, *B, X.lad_row0, X.lad_n,
