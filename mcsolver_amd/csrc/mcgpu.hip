@@ -2135,12 +2135,29 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
 #endif
 static constexpr int PASS_GPT = MCG_PASS_GPT;
 static constexpr int PASS_PAT_INTS = 4096;  // LDS for pattern rows (16 KB)
+// Workgroups are dealt to the 8 XCDs in turn (workgroup b to XCD b % 8),
+// and each XCD has its own L2.  xcd_item maps workgroup b to a work item so
+// that every XCD takes one contiguous run of the `total` items: the items of
+// a run are neighbouring chunks of a replica, whose neighbour sites then
+// come from that XCD's L2 instead of being fetched again by another XCD.
+// The grid has 8 * ceil(total / 8) workgroups; -1: no item.
+static constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_item(int b, int total) {
+  const int per = (total + NXCD - 1) / NXCD;
+  const int w = (b % NXCD) * per + b / NXCD;
+  return w < total ? w : -1;
+}
+__host__ __device__ inline int xcd_grid(int total) { return (total + NXCD - 1) / NXCD * NXCD; }
+
 template <int NC, typename ST>
 __global__ __launch_bounds__(256) void k_pass(mcg_batch B, const int32_t* __restrict__ rep_list, int colour,
-                                              int64_t sweep) {
+                                              int64_t sweep, int nchunk, int nitems) {
   __shared__ int32_t s_pat[PASS_PAT_INTS];
   __shared__ double s_red[4 * 4];
-  const int rep = rep_list[blockIdx.y];
+  const int item = xcd_item(blockIdx.x, nitems);
+  if (item < 0) return;  // the whole workgroup
+  const int chunk = item % nchunk;
+  const int rep = rep_list[item / nchunk];
   const mcg_replica rp = B.reps[rep];
   const mcg_graph G = B.graphs[rp.graph];
   const int zmax = G.zmax;
@@ -2165,7 +2182,7 @@ __global__ __launch_bounds__(256) void k_pass(mcg_batch B, const int32_t* __rest
     const GAS int32_t* lcid = G.lcid ? gas(G.lcid) : nullptr;
 #pragma unroll MCG_PASS_UNROLL
     for (int u = 0; u < PASS_GPT; ++u) {
-      const int q = (blockIdx.x * PASS_GPT + u) * (int)blockDim.x + (int)threadIdx.x;
+      const int q = (chunk * PASS_GPT + u) * (int)blockDim.x + (int)threadIdx.x;
       if (q >= ngrp) break;
       if (pats) {
         if (G.ucid >= 0)
@@ -2254,8 +2271,10 @@ __global__ __launch_bounds__(1024) void k_measure(mcg_batch B, const int32_t* __
 static constexpr int FSLOTS = FSUMS + 8;  // per replica: frame sums, then NREC recompute sums
 template <int NC, typename ST>
 __global__ __launch_bounds__(256) void k_frame_part(mcg_batch B, const int32_t* __restrict__ rep_list, int mode,
-                                                    int64_t frame, double* __restrict__ fsum) {
-  const int rep = rep_list[blockIdx.y];
+                                                    int64_t frame, double* __restrict__ fsum, int np, int nitems) {
+  const int item = xcd_item(blockIdx.x, nitems);
+  if (item < 0) return;  // the whole workgroup
+  const int rep = rep_list[item / np];
   const mcg_replica rp = B.reps[rep];
   const mcg_graph G = B.graphs[rp.graph];
   __shared__ double s_red[4 * RK];
@@ -2263,7 +2282,7 @@ __global__ __launch_bounds__(256) void k_frame_part(mcg_batch B, const int32_t* 
   const Spins<ST> sp{(ST*)B.spins + rp.spin_off, G.n};
   const int row = rp.row0 + B.rung[rep];
   const float h = B.row_h[row];
-  const int part = blockIdx.x, np = gridDim.x;
+  const int part = item % np;
   double* out = fsum + (size_t)rep * FSLOTS;
   if (mode & 1) {
     double r[NREC];
@@ -3122,9 +3141,11 @@ int launch_pass(const mcg_batch* B, const int32_t* dreps, int nreps, int maxcnt,
                 hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   if (nreps == 0) return 0;
   const int ngrp = (maxcnt + SitesPerDraw<NC>::G - 1) / SitesPerDraw<NC>::G;
-  const unsigned chunks = (unsigned)std::max(1, (ngrp + 256 * PASS_GPT - 1) / (256 * PASS_GPT));
-  hipExtLaunchKernelGGL((k_pass<NC, ST>), dim3(chunks, (unsigned)nreps), dim3(256), 0, s, e0, e1, 0, *B, dreps,
-                        colour, sweep);
+  const int chunks = std::max(1, (ngrp + 256 * PASS_GPT - 1) / (256 * PASS_GPT));
+  const int64_t items = (int64_t)chunks * nreps;
+  if (items > (1 << 30)) return fail("streaming pass: %lld work items", (long long)items);
+  hipExtLaunchKernelGGL((k_pass<NC, ST>), dim3((unsigned)xcd_grid((int)items)), dim3(256), 0, s, e0, e1, 0, *B, dreps,
+                        colour, sweep, chunks, (int)items);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -3150,8 +3171,10 @@ int launch_frame(const mcg_batch* B, const int32_t* dreps, int nreps, int mode, 
                  hipStream_t s) {
   if (nreps == 0) return 0;
   if (mode & 5) {
-    hipLaunchKernelGGL((k_frame_part<NC, ST>), dim3((unsigned)np, (unsigned)nreps), dim3(256), 0, s, *B, dreps, mode,
-                       frame, fsum);
+    const int64_t items = (int64_t)np * nreps;
+    if (items > (1 << 30)) return fail("split measurement: %lld work items", (long long)items);
+    hipLaunchKernelGGL((k_frame_part<NC, ST>), dim3((unsigned)xcd_grid((int)items)), dim3(256), 0, s, *B, dreps,
+                       mode, frame, fsum, np, (int)items);
     HIPCHK(hipGetLastError());
   }
   hipLaunchKernelGGL((k_frame_end<NC, ST>), dim3((unsigned)nreps), dim3(64), 0, s, *B, dreps, mode, frame, fsum);
