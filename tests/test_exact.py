@@ -60,18 +60,24 @@ def test_oracle_ring_energy(model, T):
     assert abs(es.mean() - want) < 4 * es.std(ddof=1) / np.sqrt(len(es)) + 2e-3, (es.mean(), want)
 
 
-def test_oracle_single_spin_anisotropy():
-    """Isolated Heisenberg spins (no bonds) with D and h: E per site and
-    <s_z> against quadrature."""
-    D, h, T = (0.1, -0.2, -0.5), 0.3, 0.8
-    Lt = lat.build_lattice(16, 16, 1, np.eye(3), [[0, 0, 0]], [], [1.0], [list(D)])
+@pytest.mark.parametrize("nc", [3, 2])
+def test_oracle_single_spin_anisotropy(nc):
+    """Isolated spins (no bonds) with D and h: E per site and the field-axis
+    spin <s_z> (Heisenberg) or <s_x> (XY: the field acts on x and the
+    reference reports the x projection, xyLib.c:203,633-654) against
+    quadrature; XY with a spin length of 1.5 and D_x != D_y."""
+    if nc == 3:
+        D, h, T, S = (0.1, -0.2, -0.5), 0.3, 0.8, 1.0
+    else:
+        D, h, T, S = (0.25, -0.15, 0.0), 0.3, 0.6, 1.5
+    Lt = lat.build_lattice(16, 16, 1, np.eye(3), [[0, 0, 0]], [], [S], [list(D)])
     es, ms = [], []
     for seed in range(6):
         a = lat.on_args(Lt, T, h, 100, 400)
-        out, _, _ = mcref.on_run(3, a, seed=seed + 3)
+        out, _, _ = mcref.on_run(nc, a, seed=seed + 3)
         es.append(out[8] * T)
         ms.append(out[25])
-    e_ex, m_ex = exact.single_spin(D, h, 1 / T)
+    e_ex, m_ex = exact.single_spin(D, h, 1 / T, nc=nc, S=S)
     es, ms = np.array(es), np.array(ms)
     assert abs(es.mean() - e_ex) < 4 * es.std(ddof=1) / np.sqrt(6) + 2e-3
     assert abs(ms.mean() - m_ex) < 4 * ms.std(ddof=1) / np.sqrt(6) + 2e-3

@@ -128,3 +128,34 @@ def test_fp16_storage_dropin_bias_small(gpu):
     f32 = [_flat_scalars(on_main(3, *a, seed=7000 + r), [8, 10, 20]) for r in range(R)]
     f16 = [_flat_scalars(on_main(3, *a, seed=8000 + r, store="f16"), [8, 10, 20]) for r in range(R)]
     _compare(f16, f32, ["E", "U4", "si_dir"], 1e-2)
+
+
+def _xy_2orb(L):
+    """Two XY orbitals per cell with spin lengths 1 and 1.5, four unequal
+    bonds, per-orbital anisotropy; the correlation pair is (orbital 0,
+    orbital 1), so the reference's M = |S_i| / nLat (the orbital-0
+    sublattice, xyLib.c:633-654) is not |total spin| / nLat."""
+    bonds = [lat.Bond(0, 1, (0, 0, 0), [-1.0, -1.0, 0.0]), lat.Bond(1, 0, (1, 0, 0), [-0.7, -0.7, 0.0]),
+             lat.Bond(0, 0, (0, 1, 0), [-0.8, -0.8, 0.0]), lat.Bond(1, 1, (0, 1, 0), [-1.2, -1.2, 0.0])]
+    return lat.build_lattice(L, L, 1, np.eye(3), [[0, 0, 0], [0.5, 0, 0]], bonds, [1.0, 1.5],
+                             [[0.1, -0.1, 0.0], [-0.05, 0.05, 0.0]], ki_s=0, ki_t=1)
+
+
+def test_xy_two_orbital_dropin_vs_oracle(gpu):
+    """XY on a two-orbital lattice with ki_s != ki_t and the field on x: the
+    sublattice observables (S_i, S_j, their projections on the total spin
+    direction and on the field axis) and E, U4 against the oracle; the
+    oracle's own S_i differs from |total spin| per site, so a sublattice
+    mistake cannot pass."""
+    X = _xy_2orb(6)
+    a = lat.on_args(X, 0.9, 0.08, 300, 3000)
+    idx = [0, 1, 3, 4, 6, 8, 9, 10, 20, 21, 22, 23, 24, 25]
+    names = ["si_x", "si_y", "sj_x", "sj_y", "sij", "E", "E2", "U4", "si_dir", "sj_dir", "stot_dir", "si_h", "sj_h",
+             "stot_h"]
+    cpu_runs = [_flat_scalars(mcref.on_tuple(2, a, seed=6100 + r), idx) for r in range(R)]
+    gpu_runs = [_flat_scalars(on_main(2, *a, seed=5100 + r), idx) for r in range(R)]
+    c = np.array(cpu_runs)
+    # orbital 0's projection per site differs from the total's (spin lengths 1 and 1.5)
+    assert abs(c[:, names.index("si_dir")].mean() - c[:, names.index("stot_dir")].mean()) > 0.1
+    assert abs(c[:, names.index("si_h")].mean() - c[:, names.index("sj_h")].mean()) > 0.1
+    _compare(gpu_runs, cpu_runs, names, 3e-3)
