@@ -3158,11 +3158,14 @@ int launch_measure(const mcg_batch* B, const int32_t* dreps, int nreps, int mode
   return 0;
 }
 
-// workgroups per replica of the split measurement: about four per CU over
-// the batch, at least 1024 sites each
+// workgroups per replica of the split measurement: about MCG_FRAME_WG_PER_CU
+// per CU over the batch, at least 1024 sites each
+#ifndef MCG_FRAME_WG_PER_CU
+#define MCG_FRAME_WG_PER_CU 16
+#endif
 int frame_parts(int nreps, int nmax, int ncu) {
   if (nreps <= 0) return 1;
-  const int want = (4 * std::max(ncu, 1) + nreps - 1) / nreps;
+  const int want = (MCG_FRAME_WG_PER_CU * std::max(ncu, 1) + nreps - 1) / nreps;
   return std::max(1, std::min({want, std::max(1, nmax / 1024), 65535}));
 }
 
