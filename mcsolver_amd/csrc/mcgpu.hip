@@ -1117,6 +1117,12 @@ struct SweepArgs {
   double* mslot;             // [2][nrep][MAX_PARTS][3] published magnetisation changes
   void* xbuf;                // [2][nslots][MAX_PARTS][max_export][NC] exported sites
   double* mprev3;            // [3][nrow] M' of each row's last frames (frame f in slot f % 3)
+  double* fpart;             // [nslots][MAX_PARTS][FSUMS] split replicas' partial frame sums
+                             // (spin-derived observables); their arrival count is sync[slot]
+  void* fspin;               // [2][nslots][NC][fspin_n] split replicas' sites as published for a
+                             // frame, by sweep parity: the next sweep's publication must not
+                             // overwrite sites a slower part still reads
+  int32_t fspin_n;
   int32_t max_export;
   int32_t nslots;
   int32_t rec_host;          // the full recompute is the host's (a launch ends at each
@@ -1310,6 +1316,45 @@ __device__ void part_copy(ST* lds_s, ST* gsp, int n, int nc, const int32_t* colp
   }
 }
 
+// the sites of part p (every colour) from LDS (component stride n) to a
+// copy of the replica in global memory (component stride dn) with
+// device-scope stores, so that any workgroup of the launch can read them
+// (DevSpins) after the next grid barrier.  The copy is not the replica's
+// spin array: other parts still stage from that at the launch start.
+template <typename ST>
+__device__ void part_publish(const ST* lds_s, typename RawBits<sizeof(ST)>::T* dst, int n, int dn, int nc,
+                             const int32_t* colptr, int ncol, int gd, int p, int np, int lt, int TPR) {
+  typedef typename RawBits<sizeof(ST)>::T U;
+  const U* lds = (const U*)lds_s;
+  for (int c = 0; c < ncol; ++c) {
+    const int b = colptr[c], cnt = colptr[c + 1] - b;
+    const int ngrp = (cnt + gd - 1) / gd;
+    int e0, e1;
+    part_range(b, cnt, ngrp, gd, p, np, e0, e1);
+    for (int k = 0; k < nc; ++k)
+      for (int e = e0 + lt; e < e1; e += TPR) dev_store(dst + (size_t)k * dn + e, lds[(size_t)k * n + e]);
+  }
+}
+
+// one replica's spins in global memory read with device-scope loads: the
+// sites other workgroups of the same launch published with part_publish
+template <typename ST>
+__device__ __forceinline__ float raw_spin(typename RawBits<sizeof(ST)>::T u);
+template <>
+__device__ __forceinline__ float raw_spin<float>(uint32_t u) { return __uint_as_float(u); }
+template <>
+__device__ __forceinline__ float raw_spin<__half>(uint16_t u) { return __half2float(__ushort_as_half(u)); }
+template <>
+__device__ __forceinline__ float raw_spin<int8_t>(uint8_t u) { return (float)(int8_t)u; }
+template <typename ST>
+struct DevSpins {
+  typedef ST value_type;
+  typedef typename RawBits<sizeof(ST)>::T U;
+  const U* p;
+  int n;
+  __device__ __forceinline__ float get(int i, int c) const { return raw_spin<ST>(dev_load(p + (size_t)c * n + i)); }
+};
+
 // ints per part in the split tables: (start, count) of the extra groups per
 // colour, then (start, count) of the export list
 __host__ __device__ inline int halo_desc_ints(int ncol) { return 2 * ncol + 2; }
@@ -1415,13 +1460,61 @@ __device__ __forceinline__ double frame_mprev3(const SweepArgs& a, int64_t f, in
 }
 template <int NC>
 __device__ __forceinline__ void take_frame(const mcg_graph& G, const mcg_batch& B, const SweepArgs& a, double* s_st,
-                                           double* s_frm, int lane, int64_t f, int row, int nrow, double mp) {
+                                           double* s_frm, int lane, int64_t f, int row, int nrow, double mp,
+                                           int flags) {
   double* F = s_frm + lane * FRM;
   double* Cn = F + FRM_ACC;
-  frame_terms<NC>(G, B, s_st + lane * MCG_NSTATE, F, B.meas & ~MEAS_SPIN_OBS, f, Cn);
+  frame_terms<NC>(G, B, s_st + lane * MCG_NSTATE, F, flags, f, Cn);
   const double Mf = Cn[MCG_A_MM];
   Cn[MCG_A_MM] = mp * Mf;
   dev_store(a.mprev3 + (size_t)(f % 3) * nrow + row, Mf);
+}
+
+// Split replica, measured frame with spin-derived observables: each part's
+// share F[0, FSUMS) of the sums (frame_sums) goes to a.fpart and counts in
+// on the replica's counter a.sync[slot]; part 0 waits for the other parts of
+// this frame (the counter counts from the run's first frame, a.frame0), then
+// adds their sums to its own in part order, so the result does not depend on
+// arrival order.  Bounded like the grid barrier.  Called by every thread of
+// the part's workgroup; false: time-out (the caller leaves the kernel).
+// A part overwrites its slot only after the next grid barrier, which part 0
+// passes after reading it.
+__device__ bool gather_frame_sums(const SweepArgs& a, const mcg_batch& B, double* F, int slot, int part, int np,
+                                  int64_t frame, int* s_flag) {
+  double* fp = a.fpart + (size_t)slot * MAX_PARTS * FSUMS;
+  uint32_t* cnt = a.sync + (size_t)slot * SYNC_STRIDE;
+  __syncthreads();  // F is written by the first thread
+  if (part != 0) {
+    if (threadIdx.x < 64)
+      for (int q = threadIdx.x; q < FSUMS; q += 64) dev_store(fp + (size_t)part * FSUMS + q, F[q]);
+    stores_done();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+  }
+  if (threadIdx.x == 0) {
+    const uint32_t want = (uint32_t)((frame - a.frame0 + 1) * (int64_t)(np - 1));
+    int ok = 1;
+    uint32_t it = 0;
+    while ((int32_t)(dev_load(cnt) - want) < 0) {
+      if (++it > (1u << 22)) {
+        atomicOr(B.err, MCG_ERR_BARRIER);
+        ok = 0;
+        break;
+      }
+    }
+    *s_flag = ok;
+  }
+  __syncthreads();
+  if (!*s_flag) return false;
+  if (threadIdx.x < 64)
+    for (int q = threadIdx.x; q < FSUMS; q += 64) {
+      double x = F[q];
+      for (int p = 1; p < np; ++p) x += dev_load(fp + (size_t)p * FSUMS + q);
+      F[q] = x;
+    }
+  __syncthreads();
+  return true;
 }
 
 // End of a persistent sweep for a workgroup of unsplit replicas: the state
@@ -1842,6 +1935,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
     // barrier always separates two frames of a row taken by different
     // workgroups.
     const bool frames_after = a.has_split || a.pt_every == 1;
+    const int fflags = FULL ? B.meas : (B.meas & ~MEAS_SPIN_OBS);  // what a frame measures
     int rep_l = 0, row0_l = 0, last_row = -1;
     if (l_grp) {
       const ItemRep irl = recs[w.off + lane];
@@ -1891,6 +1985,11 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
           }
           __syncthreads();
         }
+        if constexpr (FULL) {
+          // the spin-derived sums of every group's frame, from its LDS
+          // replica (the frame area is free after the tail)
+          if (meas) frame_sums<NC, Spins<ST>, true>(sp, C, B, fcur, key0, key1, hg, 0, 1);
+        }
       } else {
         // one replica: lane 0 of the first wave gets the part's sums
         wave0_sums<NC>(s_red, TPR, fE, fM, v);
@@ -1903,9 +2002,16 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
         U* xo = xb + (((size_t)par * a.nslots + w.slot) * MAX_PARTS + pi.part) * (size_t)a.max_export * NC;
         for (int i = lt; i < nex; i += TPR)
           export_site<NC>(xo + (size_t)i * NC, lraw, n, ex[i]);
+        if constexpr (FULL) {
+          // the frame reads every part's sites: publish this part's own
+          if (meas)
+            part_publish<ST>(my, (U*)a.fspin + ((size_t)par * a.nslots + w.slot) * NC * a.fspin_n, n, a.fspin_n,
+                             NC, s_colptr, G.ncol, GD, pi.part, pi.np, lt, TPR);
+        }
       }
       if (meas && l_grp && !frames_after) {
-        take_frame<NC>(G, B, a, s_st, s_frm, lane, fcur, rowg_l, nrow, frame_mprev3<NC>(a, fcur, rowg_l, nrow));
+        take_frame<NC>(G, B, a, s_st, s_frm, lane, fcur, rowg_l, nrow, frame_mprev3<NC>(a, fcur, rowg_l, nrow),
+                       fflags);
         last_row = rowg_l;
       }
       // the swap words of the first ladder are drawn while the barrier
@@ -1960,10 +2066,19 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
           }
         }
       }
+      if constexpr (FULL) {
+        if (meas && pi.np > 1) {
+          // this part's share of the spin-derived sums, over every part's
+          // published sites; part 0 gathers the others'
+          const DevSpins<ST> dsp{(const U*)a.fspin + ((size_t)par * a.nslots + w.slot) * NC * a.fspin_n, a.fspin_n};
+          frame_sums<NC, DevSpins<ST>, true>(dsp, C, B, fcur, key0, key1, hg, pi.part, pi.np);
+          if (!gather_frame_sums(a, B, s_frm, w.slot, pi.part, pi.np, fcur, s_flag)) return;
+        }
+      }
       // the frame of every replica (first wave, lane g), while the other
       // waves run the tempering pass
       if (meas && l_grp && frames_after) {
-        take_frame<NC>(G, B, a, s_st, s_frm, lane, fcur, rowg_l, nrow, mp);
+        take_frame<NC>(G, B, a, s_st, s_frm, lane, fcur, rowg_l, nrow, mp, fflags);
         last_row = rowg_l;
       }
       if (pt_now) {
@@ -2022,7 +2137,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
       }
       if (meas && nfr > 0) {
         __syncthreads();
-        if (g < nfr) flush_frame<NC>(C, B, rowg, B.meas & ~MEAS_SPIN_OBS);
+        if (g < nfr) flush_frame<NC>(C, B, rowg, fflags);
       }
     }
     __syncthreads();
@@ -2689,7 +2804,8 @@ struct RunCtx {
   // offsets of the pieces, then their device addresses
   size_t o_items = 0, o_item_reps = 0, o_stream = 0, o_all = 0, o_lrow0 = 0, o_ln = 0, o_halo = 0, o_repnp = 0,
          o_blklad = 0, o_sync = 0, o_xpart = 0, o_bar = 0, o_eslot = 0, o_mslot = 0, o_xbuf = 0, o_mprev3 = 0,
-         o_fsum = 0;
+         o_fsum = 0, o_fpart = 0, o_fspin = 0;
+  int fspin_n = 0;  // component stride of the published-site copies
   const WorkItem* items = nullptr;
   const ItemRep* item_reps = nullptr;
   const int32_t *stream_reps = nullptr, *all_reps = nullptr, *lad_row0 = nullptr, *lad_n = nullptr,
@@ -2698,6 +2814,8 @@ struct RunCtx {
   uint32_t *sync = nullptr, *bar = nullptr;
   double *xpart = nullptr, *eslot = nullptr, *mslot = nullptr, *mprev3 = nullptr;
   double* fsum = nullptr;  // split measurement: FSLOTS per replica, zero between frames
+  double* fpart = nullptr;  // persistent split replicas: partial frame sums of each part
+  void* fspin = nullptr;    // persistent split replicas: sites published for a frame
   void* xbuf = nullptr;
 };
 
@@ -2919,8 +3037,14 @@ int plan_lds(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool force_stream
   std::vector<char> nosplit(B->ngraph, 0);
   // automatic splits only where the launch can be persistent: otherwise a
   // launch with split replicas runs one sweep and restages them every sweep,
-  // which costs more than the imbalance it removes
-  const bool auto_split = !(B->meas & MEAS_SPIN_OBS) && !P->no_persist;
+  // which costs more than the imbalance it removes.  With spin-derived
+  // observables a persistent launch needs pattern links (modes 0 and 3).
+  // Only two-colour graphs: a part recomputes every group its own sites
+  // depend on through lower colours, a cone that widens by a link per
+  // colour (the 6-colour honeycomb with second neighbours: 34.1 us per
+  // sweep unsplit, 36.4 us in 2 parts, 36.3 us in 4).
+  const bool auto_split = !P->no_persist;
+  const bool spin_obs = (B->meas & MEAS_SPIN_OBS) != 0;
   struct Split {
     int gi, rep, np;
   };
@@ -2943,7 +3067,8 @@ int plan_lds(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool force_stream
         int np = 1;
         if (P->parts >= 2)
           np = P->parts;
-        else if (P->parts == 0 && auto_split && !nosplit[gi] && G.n >= 8192 && G.n > t)
+        else if (P->parts == 0 && auto_split && !nosplit[gi] && G.n >= 8192 && G.n > t && G.ncol <= 2 &&
+                 (!spin_obs || lds_mode(G) == 0 || lds_mode(G) == 3))
           np = std::min({MAX_PARTS, (int)std::ceil(G.n / t), G.n / 4096});
         if (np >= 2) {
           gnp[gi] = np;
@@ -3077,7 +3202,10 @@ int launch_lds(const mcg_batch* B, RunCtx& X, const SweepArgs& a, hipStream_t s,
     hipEvent_t e0, e1;
     if (T.pair(&e0, &e1)) return -1;
     int r;
-    if (X.persist)  // single mode, no spin-derived observables (checked when planned)
+    if (X.persist && full)  // single pattern mode (checked when planned)
+      r = m == 0 ? launch_lds_range<NC, ST, 0, true, true>(B, X, a, s, i0, i1, e0, e1)
+                 : launch_lds_range<NC, ST, 3, true, true>(B, X, a, s, i0, i1, e0, e1);
+    else if (X.persist)  // single mode (checked when planned)
       r = m == 0   ? launch_lds_range<NC, ST, 0, false, true>(B, X, a, s, i0, i1, e0, e1)
           : m == 1 ? launch_lds_range<NC, ST, 1, false, true>(B, X, a, s, i0, i1, e0, e1)
           : m == 2 ? launch_lds_range<NC, ST, 2, false, true>(B, X, a, s, i0, i1, e0, e1)
@@ -3101,10 +3229,10 @@ int launch_lds(const mcg_batch* B, RunCtx& X, const SweepArgs& a, hipStream_t s,
 }
 
 // Can this run use a persistent launch?  All replicas on the LDS path, one
-// link mode, no spin-derived observables, tempering or split replicas to
-// synchronise (otherwise a plain multi-sweep launch already is persistent),
-// the tempering state fits in LDS,
-// and every workgroup resident at once.
+// link mode (a pattern mode when spin-derived observables are measured),
+// tempering or split replicas to synchronise (otherwise a plain multi-sweep
+// launch already is persistent), the tempering state fits in LDS, and every
+// workgroup resident at once.
 template <int NC, typename ST>
 int check_persist(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool* ok) {
   *ok = false;
@@ -3113,16 +3241,20 @@ int check_persist(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool* ok) {
   for (int q = 0; q < 4; ++q)
     if (X.mode_begin[q + 1] > X.mode_begin[q]) ++modes, m = q;
   if (modes != 1) return 0;
-  if (B->meas & MEAS_SPIN_OBS) return 0;
+  const bool full = (B->meas & MEAS_SPIN_OBS) != 0;
+  if (full && m != 0 && m != 3) return 0;
   const bool pt = P->pt_every > 0 && !X.hl_row0.empty();
   if (!pt && X.nslots == 0) return 0;
   const size_t bytes = lds_bytes_for<NC, ST>(B, X, 0, (int)X.hitems.size()) + persist_lds_bytes(B->nrow, B->nrep);
   if (bytes > X.lds_max) return 0;
-  const void* f = m == 0   ? (const void*)k_lds<NC, ST, 0, false, true>
+  const void* f = full ? (m == 0 ? (const void*)k_lds<NC, ST, 0, true, true> : (const void*)k_lds<NC, ST, 3, true, true>)
+                  : m == 0 ? (const void*)k_lds<NC, ST, 0, false, true>
                   : m == 1 ? (const void*)k_lds<NC, ST, 1, false, true>
                   : m == 2 ? (const void*)k_lds<NC, ST, 2, false, true>
                            : (const void*)k_lds<NC, ST, 3, false, true>;
-  if (m == 0) set_lds_attr<NC, ST, 0, false, true>(bytes);
+  if (full && m == 0) set_lds_attr<NC, ST, 0, true, true>(bytes);
+  else if (full) set_lds_attr<NC, ST, 3, true, true>(bytes);
+  else if (m == 0) set_lds_attr<NC, ST, 0, false, true>(bytes);
   else if (m == 1) set_lds_attr<NC, ST, 1, false, true>(bytes);
   else if (m == 2) set_lds_attr<NC, ST, 2, false, true>(bytes);
   else set_lds_attr<NC, ST, 3, false, true>(bytes);
@@ -3368,8 +3500,16 @@ int prepare_typed(const mcg_batch* B, const mcg_plan* P, RunCtx& X) {
     X.o_mslot = take(&off, (size_t)2 * B->nrep * MAX_PARTS * 3 * 8);
     X.o_xbuf = take(&off, (size_t)2 * std::max(X.nslots, 1) * MAX_PARTS * std::max(X.max_export, 1) * NC * sizeof(ST));
     X.o_mprev3 = take(&off, (size_t)3 * B->nrow * 8);
+    X.o_fpart = take(&off, (size_t)std::max(X.nslots, 1) * MAX_PARTS * FSUMS * 8);
   }
   X.reset_bytes = off - X.reset_off;
+  if (X.persist && X.nslots > 0 && (B->meas & MEAS_SPIN_OBS)) {
+    // written in full before every read: outside the reset region
+    int nmax = 0;
+    for (int gi = 0; gi < B->ngraph; ++gi) nmax = std::max(nmax, B->graphs_host[gi].n);
+    X.fspin_n = (int)((nmax + 63) / 64 * 64);
+    X.o_fspin = take(&off, (size_t)2 * X.nslots * NC * X.fspin_n * sizeof(ST));
+  }
   X.dev_bytes = off;
   X.hstage.assign(X.tab_bytes, 0);
   auto put = [&](size_t o, const void* src, size_t n) {
@@ -3427,6 +3567,8 @@ int bind_region(RunCtx& X, unsigned char* dev, hipStream_t s) {
   X.xbuf = at(X.o_xbuf);
   X.mprev3 = (double*)at(X.o_mprev3);
   X.fsum = (double*)at(X.o_fsum);
+  X.fpart = (double*)at(X.o_fpart);
+  X.fspin = at(X.o_fspin);
   if (X.tab_bytes && hipMemcpyAsync(dev, X.hstage.data(), X.tab_bytes, hipMemcpyHostToDevice, s) != hipSuccess)
     return fail("hipMemcpyAsync of the run tables failed");
   return 0;
@@ -3529,6 +3671,9 @@ int exec_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s, RunCtx& X) 
       a.mslot = X.mslot;
       a.xbuf = X.xbuf;
       a.mprev3 = X.mprev3;
+      a.fpart = X.fpart;
+      a.fspin = X.fspin;
+      a.fspin_n = X.fspin_n;
       a.max_export = std::max(X.max_export, 1);
       a.nslots = std::max(X.nslots, 1);
       a.rec_host = rec_host ? 1 : 0;
