@@ -155,6 +155,16 @@ typedef struct mcg_batch {
   int32_t *wlist;          /* [sum n] Wolff cluster members (or NULL) */
   const mcg_graph *graphs_host;  /* host mirror of graphs (launch planning) */
   const mcg_replica *reps_host;  /* host mirror of reps */
+  /* Exact Ising acceptance (NULL: FP32 exp(-beta dE) against a 24-bit
+   * uniform).  When every folded coupling J*S_target, field and spin length
+   * is a multiple of a power of two small enough that the local field and dE
+   * are exact in FP32, dE = k e_unit with 0 < k < nthr for every uphill flip,
+   * and the flip is accepted when the uint32 draw is below
+   * ithr[row*nthr + k] = floor(2^32 exp(-row_beta[row] k e_unit))
+   * (isingLib.c:238-254 accepts when exp(corr) > rand()/RAND_MAX). */
+  const uint32_t *ithr;    /* [nrow*nthr] device */
+  int32_t nthr;
+  float e_unit_inv;        /* 1 / e_unit (a power of two) */
 } mcg_batch;
 
 /* run plan for mcg_run */

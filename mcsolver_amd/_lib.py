@@ -48,7 +48,8 @@ class Batch(C.Structure):
                [("seed_lo", C.c_uint32), ("seed_hi", C.c_uint32), ("recompute_every", C.c_int32),
                 ("pad_", C.c_int32)] + \
                [(k, P) for k in ("graphs", "reps", "spins", "row_beta", "row_h", "rung", "who", "state", "acc",
-                                 "bins", "gacc", "pt_acc", "err", "wstamp", "wlist", "graphs_host", "reps_host")]
+                                 "bins", "gacc", "pt_acc", "err", "wstamp", "wlist", "graphs_host", "reps_host")] + \
+               [("ithr", P), ("nthr", C.c_int32), ("e_unit_inv", C.c_float)]
 
 
 class Plan(C.Structure):

@@ -178,6 +178,29 @@ struct PatOffLinks {
 
 static constexpr double REF_PI = 3.1415926535;  // heisenbergLib.c:6
 
+// The acceptance of a row: beta, and for exact Ising the row's threshold
+// table thr[k] = floor(2^32 exp(-beta k u)) for dE = k u (u = 1 / inv_unit, a
+// power of two; thr0 / nthr: the whole table, for replicas of other rows).
+// A float converts to an Acc without a table (the FP32 exp path).
+struct Acc {
+  float beta;
+  const uint32_t* thr;
+  float inv_unit;
+  const uint32_t* thr0;
+  int nthr;
+  __device__ __forceinline__ Acc(float b) : beta(b), thr(nullptr), inv_unit(0.0f), thr0(nullptr), nthr(0) {}
+  __device__ __forceinline__ Acc(float b, const uint32_t* t, float iu, const uint32_t* t0, int n)
+      : beta(b), thr(t), inv_unit(iu), thr0(t0), nthr(n) {}
+  // the same table at another row (beta b)
+  __device__ __forceinline__ Acc at(float b, int row) const {
+    return Acc(b, thr0 ? thr0 + (size_t)row * nthr : nullptr, inv_unit, thr0, nthr);
+  }
+};
+// the acceptance of row `row` of the batch
+__device__ __forceinline__ Acc acc_of(const mcg_batch& B, float beta, int row) {
+  return B.ithr ? Acc(beta, B.ithr + (size_t)row * B.nthr, B.e_unit_inv, B.ithr, B.nthr) : Acc(beta);
+}
+
 // ---------------------------------------------------------------------------
 // the single-site Metropolis update
 // ---------------------------------------------------------------------------
@@ -195,8 +218,9 @@ static constexpr double REF_PI = 3.1415926535;  // heisenbergLib.c:6
 // each an exact +0 to dE, are skipped.
 template <int NC, typename SV, typename LK, typename CP, bool UNI = false>
 __device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, int zmax, CP coup, int ci, CP clsdat,
-                                           float beta, float h, uint32_t w_a, uint32_t w_b, uint32_t w_c,
+                                           Acc ac, float h, uint32_t w_a, uint32_t w_b, uint32_t w_c,
                                            float& dEacc, float* dMacc, bool aniso = true) {
+  const float beta = ac.beta;
   const float S = clsdat[ci * 8 + 0];
   if constexpr (NC == 1) {
     float f = 0.0f;
@@ -234,8 +258,17 @@ __device__ __forceinline__ void metro_core(const SV& sp, int i, const LK& lk, in
     const float s = sg * S;
     const float t = f - h;
     const float dE = (-2.0f * s) * t;
-    const float db = dE * beta;
-    if (db <= 0.0f || u24(w_c) < exp_neg(-db)) {
+    bool acc;
+    if (ac.thr) {
+      // exact: couplings, fields and spin lengths are multiples of powers of
+      // two small enough that every operation above is exact, dE = k u with
+      // 0 < k < nthr, and thr[k] = floor(2^32 exp(-beta k u))
+      acc = dE <= 0.0f || w_c < ac.thr[(int)(dE * ac.inv_unit)];
+    } else {
+      const float db = dE * beta;
+      acc = db <= 0.0f || u24(w_c) < exp_neg(-db);
+    }
+    if (acc) {
       sp.set(i, 0, -sg);
       dEacc += dE;
       dMacc[0] += -2.0f * s;
@@ -383,7 +416,7 @@ template <int NC, bool UNI = false, typename SV, typename CP>
 __device__ __forceinline__ void metro_group(const SV& sp, int b, int cnt, int q, int colour,
                                             const GAS uint32_t* __restrict__ link,
                                             const GAS int32_t* __restrict__ lcid, int zmax, CP coup,
-                                            const GAS int32_t* __restrict__ cls, CP clsdat, float beta, float h,
+                                            const GAS int32_t* __restrict__ cls, CP clsdat, Acc beta, float h,
                                             uint64_t sweep, uint32_t key0, uint32_t key1, float& dEacc,
                                             float* dMacc) {
   constexpr int G = SitesPerDraw<NC>::G;
@@ -410,7 +443,7 @@ __device__ __forceinline__ void metro_group(const SV& sp, int b, int cnt, int q,
 template <int NC, typename ST, bool UNI = false>
 __device__ __forceinline__ void metro_group_pat(const Spins<ST>& sp, int b, int cnt, int q, int colour,
                                                 const uint8_t* pat, const int32_t* pattab, int zmax,
-                                                const float* coup, const float* clsdat, float beta, float h,
+                                                const float* coup, const float* clsdat, Acc beta, float h,
                                                 uint64_t sweep, uint32_t key0, uint32_t key1, float& dEacc,
                                                 float* dMacc, bool aniso = true) {
   constexpr int G = SitesPerDraw<NC>::G;
@@ -440,7 +473,7 @@ __device__ __forceinline__ void metro_group_pat(const Spins<ST>& sp, int b, int 
 template <int NC, typename ST, bool UNI>
 __device__ __forceinline__ void metro_group_gpat(const GSpins<ST>& sp, int b, int cnt, int q, int colour,
                                                  const GAS uint8_t* __restrict__ pat, const int32_t* pattab, int zmax,
-                                                 const GAS float* coup, const GAS float* clsdat, float beta, float h,
+                                                 const GAS float* coup, const GAS float* clsdat, Acc beta, float h,
                                                  uint64_t sweep, uint32_t key0, uint32_t key1, float& dEacc,
                                                  float* dMacc, bool aniso) {
   constexpr int G = SitesPerDraw<NC>::G;
@@ -1632,7 +1665,7 @@ __device__ __forceinline__ void lds_sweep_tail(const SV& sv, const mcg_graph& G,
 template <int NC, typename ST, int MODE>
 __device__ __forceinline__ void sweep_colours(const mcg_graph& G, const Spins<ST>& sp, const float* coup,
                                               const float* clsdat, const int32_t* s_colptr, const uint8_t* s_pid,
-                                              const int32_t* s_pat, int lt, int TPR, float beta, float h, uint64_t kk,
+                                              const int32_t* s_pat, int lt, int TPR, Acc beta, float h, uint64_t kk,
                                               uint32_t key0, uint32_t key1, const PartInfo& pi, float& fE, float* fM,
                                               bool aniso, const PoolInfo<ST>& pl, bool pfresh) {
   constexpr int GD = SitesPerDraw<NC>::G;
@@ -1668,7 +1701,8 @@ __device__ __forceinline__ void sweep_colours(const mcg_graph& G, const Spins<ST
         const float4 pv = pl.par[pl.pr];
         float pe = 0.0f, pm[3] = {0.0f, 0.0f, 0.0f};
         const Spins<ST> psp{pl.spin0 + (size_t)pl.sstride * pl.pr, G.n};
-        metro_group_pat<NC, ST, UNI>(psp, b, cnt, qm + pl.pq, c, s_pid, s_pat, zmax, cp, clsdat, pv.x, pv.y, kk, key0,
+        metro_group_pat<NC, ST, UNI>(psp, b, cnt, qm + pl.pq, c, s_pid, s_pat, zmax, cp, clsdat,
+                                     beta.at(pv.x, __float_as_int(pv.w)), pv.y, kk, key0,
                                      __float_as_uint(pv.z), pe, pm, aniso);
         // the first pool group after a tail stores, later ones add
         double* q = pl.pp + threadIdx.x;
@@ -1746,7 +1780,8 @@ __device__ __forceinline__ void lds_sweeps(const mcg_graph& G, const mcg_batch& 
     double mprev = 0.0;  // read ahead, so that the frame does not wait for it
     if (meas && !split && lt == MCG_A_MM) mprev = frame_mprev(B, row);
     float fE = 0.0f, fM[3] = {0.0f, 0.0f, 0.0f};  // this sweep's accepted changes
-    sweep_colours<NC, ST, MODE>(G, sp, coup, clsdat, s_colptr, s_pid, s_pat, lt, TPR, beta, h, kk, key0, key1, pi, fE,
+    sweep_colours<NC, ST, MODE>(G, sp, coup, clsdat, s_colptr, s_pid, s_pat, lt, TPR, acc_of(B, beta, row), h, kk, key0,
+                                key1, pi, fE,
                                 fM, aniso, pl, pfresh);
     pfresh = false;
     dE += (double)fE;
@@ -1835,7 +1870,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
   stage<ST, true>(my, gsp, NC * n, lt, TPR);
   double* st = s_st + g * MCG_NSTATE;
   if (lt < MCG_NSTATE) st[lt] = B.state[(size_t)rep * MCG_NSTATE + lt];
-  if (lt == 0) s_par[g] = make_float4(beta, h, __uint_as_float(B.seed_hi ^ (uint32_t)ir.serial), 0.0f);
+  if (lt == 0) s_par[g] = make_float4(beta, h, __uint_as_float(B.seed_hi ^ (uint32_t)ir.serial), __int_as_float(row));
   __syncthreads();
 
   const uint32_t key0 = B.seed_lo, key1 = B.seed_hi ^ (uint32_t)ir.serial;
@@ -1958,11 +1993,12 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
       const int rowg_l = row0_l + s_rung[rep_l];
       const int64_t fcur = (k - a.nthermal + 1) / a.spf - 1 + a.frame0;
       if (pl.rem > 0) {  // the pool's parameters of this sweep (rows move with tempering)
-        if (lt == 0) s_par[g] = make_float4(bg, hg, __uint_as_float(key1), 0.0f);
+        if (lt == 0) s_par[g] = make_float4(bg, hg, __uint_as_float(key1), __int_as_float(rowg));
         __syncthreads();
       }
       float fE = 0.0f, fM[3] = {0.0f, 0.0f, 0.0f};
-      sweep_colours<NC, ST, MODE>(G, sp, C.coup, C.clsdat, s_colptr, s_pid, s_pat, lt, TPR, bg, hg, kk, key0, key1, pi,
+      sweep_colours<NC, ST, MODE>(G, sp, C.coup, C.clsdat, s_colptr, s_pid, s_pat, lt, TPR, acc_of(B, bg, rowg), hg, kk,
+                                  key0, key1, pi,
                                   fE, fM, aniso, pl, true);
       double v[RK] = {(double)fE, (double)fM[0], (double)fM[1], (double)fM[2], 0.0, 0.0, 0.0, 0.0};
       double* es = a.eslot + ((size_t)par * nrepb + rep) * MAX_PARTS;
@@ -2291,7 +2327,8 @@ __global__ __launch_bounds__(256) void k_pass(mcg_batch B, const int32_t* __rest
     const int b = G.col_ptr[colour], cnt = G.col_ptr[colour + 1] - b;
     const int ngrp = (cnt + SitesPerDraw<NC>::G - 1) / SitesPerDraw<NC>::G;
     const int row = rp.row0 + B.rung[rep];
-    const float beta = B.row_beta[row], h = B.row_h[row];
+    const float h = B.row_h[row];
+    const Acc beta = acc_of(B, B.row_beta[row], row);
     const uint32_t key1 = B.seed_hi ^ rp.serial;
     const GSpins<ST> sp{gas((ST*)B.spins + rp.spin_off), G.n};
     const GAS int32_t* lcid = G.lcid ? gas(G.lcid) : nullptr;

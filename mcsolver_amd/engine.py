@@ -299,6 +299,55 @@ def upload_graph(g: GraphTables, device) -> None:
 _STORE = {"f32": (L.STORE_F32, torch.float32), "f16": (L.STORE_F16, torch.float16), "i8": (L.STORE_I8, torch.int8)}
 
 
+def _pow2_exponent(x: np.ndarray, kmax: int) -> Optional[int]:
+    """Smallest k <= kmax with every x * 2^k an integer, or None."""
+    x = np.asarray(x, np.float64).ravel()
+    for k in range(kmax + 1):
+        y = x * 2.0 ** k
+        if np.all(y == np.round(y)):
+            return k
+    return None
+
+
+def exact_ising_table(graphs: List["GraphTables"], beta: Sequence[float], h: Sequence[float],
+                      max_entries: int = 1 << 16):
+    """The exact Ising acceptance of a batch (include/mcgpu.h, mcg_batch.ithr),
+    or None.  When every folded coupling J*S_target and every field is a
+    multiple of 2^-kf (kf <= 8) and every spin length a multiple of 2^-ks
+    (ks <= 4), the sweep's FP32 local field f = sum J S_t sigma_j and
+    dE = -2 sigma_i S_i (f - h) are exact multiples of u = 2^(1-kf-ks), and a
+    flip with dE = k u > 0 is accepted when the uint32 draw is below
+    thr[row, k] = floor(2^32 exp(-beta_row k u)), beta_row the FP32 inverse
+    temperature the kernel holds.  Returns (thr [nrow, nthr] uint32, 1/u);
+    None when the inputs are not such multiples or the largest |dE|/u needs
+    more than max_entries entries per row (the sweep then keeps the FP32
+    exp(-beta dE) acceptance).  isingLib.c:238-254 accepts when
+    exp(corr) > rand()/RAND_MAX."""
+    J = np.concatenate([g.coup.reshape(-1, 9)[:, 0] for g in graphs]).astype(np.float32).astype(np.float64)
+    S = np.concatenate([g.clsdat.reshape(-1, 8)[:, 0] for g in graphs]).astype(np.float32).astype(np.float64)
+    hh = np.asarray(h, np.float32).astype(np.float64)
+    kf = _pow2_exponent(np.concatenate([J, hh]), 8)
+    ks = _pow2_exponent(S, 4)
+    if kf is None or ks is None:
+        return None
+    unit = 2.0 ** (1 - kf - ks)
+    hmax = float(np.abs(hh).max()) if hh.size else 0.0
+    kmax = 0
+    for g in graphs:
+        cid = g.lcid.reshape(g.n, -1) if g.lcid is not None else (g.link.reshape(g.n, -1) >> 24)
+        cj = g.coup.reshape(-1, 9)[:, 0].astype(np.float64)
+        fsum = np.abs(cj[cid.astype(np.int64)]).sum(1) if g.zmax else np.zeros(g.n)
+        sa = np.abs(g.clsdat.reshape(-1, 8)[:, 0].astype(np.float64)[g.cls])
+        kmax = max(kmax, int(np.ceil(np.max(2.0 * sa * (fsum + hmax)) / unit)))
+    nthr = kmax + 1
+    if nthr > max_entries:
+        return None
+    b32 = np.asarray(beta, np.float32).astype(np.float64)
+    k = np.arange(nthr, dtype=np.float64)
+    thr = np.floor(np.exp(-b32[:, None] * k[None, :] * unit) * 2.0 ** 32)
+    return np.minimum(thr, 2.0 ** 32 - 1).astype(np.uint32), 1.0 / unit
+
+
 @dataclass
 class ReplicaSpec:
     graph: int
@@ -314,7 +363,7 @@ class Batch:
     def __init__(self, nc: int, graphs: List[GraphTables], reps: List[ReplicaSpec], beta: Sequence[float],
                  h: Sequence[float], init_spins: List[np.ndarray], store: str = "f32", meas: int = 0,
                  nbins: int = 32, nframes: int = 1, seed: int = 0x5EED2026, wolff: bool = False,
-                 recompute_every: int = 0, device="cuda", serial_base: int = 0):
+                 recompute_every: int = 0, device="cuda", serial_base: int = 0, exact_ising: bool = True):
         self.nc = nc
         self.device = torch.device(device)
         if nc == 1:
@@ -401,12 +450,30 @@ class Batch:
         b.wlist = self.wlist.data_ptr() if wolff else None
         b.graphs_host = C.addressof(self.h_graphs)
         b.reps_host = C.addressof(self.h_reps)
+        # exact Ising acceptance where the inputs allow it (exact_ising_table)
+        self.ithr, self.e_unit = None, None
+        tab = exact_ising_table(graphs, beta, h) if (nc == 1 and exact_ising) else None
+        if tab is not None:
+            thr, inv_unit = tab
+            self.ithr = torch.from_numpy(thr.view(np.int32).reshape(-1).copy()).to(self.device)
+            self.e_unit = 1.0 / inv_unit
+            self._ithr_host = (thr, inv_unit)
+            b.ithr, b.nthr, b.e_unit_inv = self.ithr.data_ptr(), thr.shape[1], inv_unit
         self.b = b
         self.frames_per_row = max(1, nframes)
         self.wolff = wolff
         self._runners = {}        # plan shape -> prepared run (mcg_runner_*)
         self._next_sweep = 0      # Philox sweep index after the last run
         self._wolff_cl_end = 0    # first Wolff cluster id not yet used
+
+    def ising_table(self, rows=None):
+        """The exact Ising acceptance this batch runs with (exact_ising_table):
+        (thresholds of `rows` (all rows by default), 1/unit), or None when it
+        uses the FP32 exp rule."""
+        if self.ithr is None:
+            return None
+        thr, inv_unit = self._ithr_host
+        return (thr if rows is None else thr[np.asarray(rows)]), inv_unit
 
     def __del__(self):
         self.close()

@@ -164,6 +164,20 @@ static void cb_free(cb_graph *g) {
 
 static float stor(const cb_graph *g, float v) { return g->store == 1 ? f16r(v) : v; }
 
+/* The engine's exact Ising acceptance (mcg_batch.ithr): thresholds
+ * [nrep][nthr] for the replicas of the following mcref_cb_sweeps calls, in
+ * their order; a flip with dE = k / inv_unit > 0 is accepted when the uint32
+ * acceptance word is below thr[k].  NULL: the FP32 exp(-beta dE) rule. */
+static const uint32_t *cb_ithr;
+static int cb_nthr;
+static float cb_inv_unit;
+static const uint32_t *cb_thr_row;
+void mcref_cb_ising_table(const uint32_t *thr, int nthr, float inv_unit) {
+  cb_ithr = thr;
+  cb_nthr = nthr;
+  cb_inv_unit = inv_unit;
+}
+
 /* spins: [nc][n] floats for one replica (Ising: sigma = +-1).
  * wa: r_z word (Heisenberg), wb: angle word, wc: acceptance word */
 static void cb_site(const cb_graph *g, float *sp, long i, float beta, float h, uint32_t wa, uint32_t wb, uint32_t wc,
@@ -188,8 +202,14 @@ static void cb_site(const cb_graph *g, float *sp, long i, float beta, float h, u
     float s = sg * S;
     float t = f - h;
     float de = (-2.0f * s) * t;
-    float db = de * beta;
-    if (db <= 0.0f || uu(wc) < cexp_neg(-db)) {
+    int acc;
+    if (cb_thr_row) {
+      acc = de <= 0.0f || wc < cb_thr_row[(int)(de * cb_inv_unit)];
+    } else {
+      float db = de * beta;
+      acc = db <= 0.0f || uu(wc) < cexp_neg(-db);
+    }
+    if (acc) {
       sp[i] = -sg;
       *dE += de;
       dM[0] += -2.0f * s;
@@ -306,6 +326,7 @@ int mcref_cb_sweeps(int nc, int store, long n, int zmax, const int *nbr, const d
   for (int r = 0; r < nrep; r++) {
     float *sp = spins + (size_t)r * nc * n;
     uint32_t k0 = seed_lo, k1 = seed_hi ^ serial[r];
+    cb_thr_row = (nc == 1 && cb_ithr) ? cb_ithr + (size_t)r * cb_nthr : NULL;
     for (long s = 0; s < nsweeps; s++) {
       uint64_t k = (uint64_t)(sweep0 + s);
       for (int c = 0; c < ncol; c++) {
@@ -328,6 +349,7 @@ int mcref_cb_sweeps(int nc, int store, long n, int zmax, const int *nbr, const d
       }
     }
   }
+  cb_thr_row = NULL;
   free(order);
   free(ptr);
   free(fill);

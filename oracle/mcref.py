@@ -38,6 +38,7 @@ def lib() -> C.CDLL:
         _l.mcref_on_run.argtypes = [I, I, L_, P, P, L_, L_, L_, I, P, P, P, L_, P, L_, P, L_, L_, P, D, D, L_, L_, P,
                                     P, P, I, I, P, P, P]
         _l.mcref_cb_sweeps.argtypes = [I, I, L_, I, P, P, P, P, I, P, I, I, P, P, P, U, U, L_, L_, P, P, P]
+        _l.mcref_cb_ising_table.argtypes = [P, I, F]
         _l.mcref_cb_energy.argtypes = [I, L_, I, P, P, P, P, I, F, P, P]
         _l.mcref_cb_pt.argtypes = [I, P, P, P, P, U, U, I, L_, P]
         _l.mcref_cb_wolff.argtypes = [I, I, L_, I, P, P, P, P, I, P, I, F, F, U, U, U, L_, L_, P, P]
@@ -141,9 +142,12 @@ def on_tuple(nc, args, seed=None):
 
 
 # ---------------------------------------------------------------- checker
-def cb_sweeps(nc, store, nbr, J, S, D, colour, beta, h, serial, seed, sweep0, nsweeps, spins, diag_only=0):
+def cb_sweeps(nc, store, nbr, J, S, D, colour, beta, h, serial, seed, sweep0, nsweeps, spins, diag_only=0,
+              ising_table=None):
     """Engine-convention coloured Metropolis on replicas of one graph (in place).
-    spins: float32 [nrep, nc, n].  Returns (dE[nrep], dM[nrep,3])."""
+    spins: float32 [nrep, nc, n].  ising_table: (thr [nrep, nthr] uint32,
+    1/unit), the engine's exact Ising acceptance for these replicas (None:
+    the FP32 exp rule).  Returns (dE[nrep], dM[nrep,3])."""
     nbr = _a(nbr, np.int32); n, z = nbr.shape
     J = _a(J, np.float64).reshape(n, z, -1)
     if J.shape[2] < 9:
@@ -154,9 +158,18 @@ def cb_sweeps(nc, store, nbr, J, S, D, colour, beta, h, serial, seed, sweep0, ns
     beta = _a(beta, np.float32); h = _a(h, np.float32); serial = _a(serial, np.uint32)
     assert spins.dtype == np.float32 and spins.flags.c_contiguous
     dE = np.zeros(nrep); dM = np.zeros((nrep, 3))
-    lib().mcref_cb_sweeps(nc, store, n, z, ptr(nbr), ptr(J), ptr(S), ptr(D), diag_only, ptr(colour),
-                          int(colour.max()) + 1, nrep, ptr(beta), ptr(h), ptr(serial), seed & 0xFFFFFFFF,
-                          (seed >> 32) & 0xFFFFFFFF, sweep0, nsweeps, ptr(spins), ptr(dE), ptr(dM))
+    thr = None
+    if ising_table is not None:
+        thr = np.ascontiguousarray(ising_table[0], np.uint32)
+        assert thr.shape[0] == nrep
+        lib().mcref_cb_ising_table(ptr(thr), thr.shape[1], float(ising_table[1]))
+    try:
+        lib().mcref_cb_sweeps(nc, store, n, z, ptr(nbr), ptr(J), ptr(S), ptr(D), diag_only, ptr(colour),
+                              int(colour.max()) + 1, nrep, ptr(beta), ptr(h), ptr(serial), seed & 0xFFFFFFFF,
+                              (seed >> 32) & 0xFFFFFFFF, sweep0, nsweeps, ptr(spins), ptr(dE), ptr(dM))
+    finally:
+        if thr is not None:
+            lib().mcref_cb_ising_table(None, 0, 0.0)
     return dE, dM
 
 

@@ -27,7 +27,9 @@ def test_library_exports_every_declared_function():
 
 
 def test_struct_sizes_match_header_layout():
-    # mcg_batch: 12 int32 (48 B) + 17 pointers
-    assert C.sizeof(L.Batch) == 48 + 17 * 8
+    # mcg_batch: 12 int32 (48 B) + 17 pointers, then the exact-Ising table
+    # pointer, its row length (int32) and 1/unit (float)
+    assert C.sizeof(L.Batch) == 48 + 17 * 8 + 8 + 4 + 4
+    assert L.Batch.ithr.offset == 48 + 17 * 8
     assert C.sizeof(L.Plan) == 64
     assert C.sizeof(L.Replica) == 40

@@ -207,6 +207,7 @@ def test_parallel_tempering_bitexact(gpu, no_persist):
     reps = [ReplicaSpec(0, 0, nrung=nT, ladder=0, rung=k) for k in range(nT)]
     b = Batch(1, [g], reps, betas, [0.0] * nT, init, seed=SEED)
     b.init()
+    assert b.ising_table() is not None  # J = 1, h = 0: the exact acceptance
     nsw = 10
     b.run(0, nsw, 1, pt_every=1, kernel=L.KERNEL_LDS, no_persist=no_persist)
     got = _gpu_spins(b, nT)
@@ -219,7 +220,7 @@ def test_parallel_tempering_bitexact(gpu, no_persist):
         for k in range(nT):
             one = np.ascontiguousarray(ref[k:k + 1])
             mcref.cb_sweeps(1, 2, nbr, J, S, D, g.colour, [betas[rung[k]]], [0.0], np.array([k], np.uint32), SEED, s,
-                            1, one)
+                            1, one, ising_table=b.ising_table([rung[k]]))
             ref[k] = one[0]
         E = np.array([mcref.cb_energy(1, nbr, J, S, D, 0.0, ref[k])[0] for k in range(nT)])
         mcref.cb_pt(betas, E, who, rung, SEED, 0, s, acc)
@@ -465,3 +466,49 @@ def test_persistent_spin_observables_match_per_launch(gpu, model, parts, pt_ever
     assert np.abs(a[1][:, L.A_SIJ]).max() > 0
     if model == "heis":
         assert np.abs(a[1][:, L.A_Q]).max() > 0
+
+
+EXACT_ISING = {
+    # J = 1, h = 0 (the configs[0] workload's couplings) and a dyadic field
+    "sq16_h0": (lambda: lat.square(16, ising=True), [0.2, 0.44, 0.6, 1.0], 0.0),
+    "sq16_h025": (lambda: lat.square(16, J=-0.75, ising=True), [0.3, 0.5], 0.25),
+    # two orbitals with spin lengths 1 and 1.5, equal couplings
+    "2orb_S15": (lambda: __import__("test_gpu_dropin")._ising_2orb_uniform(6), [0.3, 0.7], 0.5),
+}
+
+
+@pytest.mark.parametrize("kernel", [L.KERNEL_LDS, L.KERNEL_STREAM, L.KERNEL_PLAIN])
+@pytest.mark.parametrize("case", sorted(EXACT_ISING))
+def test_exact_ising_bitexact(gpu, case, kernel):
+    """Dyadic couplings, fields and spin lengths: the sweep accepts through
+    the uint32 threshold table (exact_ising_table), and so does the checker;
+    spins bit for bit, and the tracked energy equals the recomputed one
+    exactly (every dE is an exact multiple of the unit)."""
+    mk, betas, h = EXACT_ISING[case]
+    Lt = mk()
+    b, g, init = _mk_batch(1, Lt, betas, h)
+    tab = b.ising_table()
+    assert tab is not None, "expected the exact acceptance"
+    nsw = 12
+    b.sweeps(0, nsw, kernel=kernel)
+    got = _gpu_spins(b, len(betas))
+    ref = _cpu_spins(1, init, "f32")
+    nbr, J, S, D = _lattice_tables(Lt, 1)
+    mcref.cb_sweeps(1, 2, nbr, J, S, D, g.colour, betas, [h] * len(betas), np.arange(len(betas), dtype=np.uint32),
+                    SEED, 0, nsw, ref, ising_table=tab)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), "spins differ"
+    st = b.state_data()
+    for k in range(len(betas)):
+        e_cpu = mcref.cb_energy(1, nbr, J, S, D, h, ref[k])
+        assert st[k, L.S_E] == e_cpu[0], (k, st[k, L.S_E], e_cpu[0])
+    # the sweep reads the table: with every threshold 0 (the T = 0 limit:
+    # only flips with dE <= 0) it follows the checker given the same table
+    b.ithr.zero_()
+    b.sweeps(nsw, nsw, kernel=kernel)
+    got = _gpu_spins(b, len(betas))
+    zero = (np.zeros_like(tab[0]), tab[1])
+    mcref.cb_sweeps(1, 2, nbr, J, S, D, g.colour, betas, [h] * len(betas), np.arange(len(betas), dtype=np.uint32),
+                    SEED, nsw, nsw, ref, ising_table=zero)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), "spins differ with the zero table"
+    for k in range(len(betas)):
+        assert mcref.cb_energy(1, nbr, J, S, D, h, ref[k])[0] <= b.state_data()[k, L.S_E] + 0.0
