@@ -117,13 +117,13 @@ def tuned_options(a, b, pt_every):
     fastest of the candidate plans timed on this batch (the batch is left as
     it was)."""
     if not a.autotune:
-        return dict(parts=a.parts, no_persist=False, reps_per_block=0)
+        return dict(parts=a.parts, no_persist=False, reps_per_block=0, block=a.block, kernel=0)
     from mcsolver_amd.autotune import autotune
     r = autotune(b, 1, pt_every, trial_sweeps=64)
     for t in r.trials:
         print("autotune %s: %s" % (t.options, "%.4f ms/sweep" % t.ms_per_sweep if t.ms_per_sweep is not None
                                     else "rejected (%s)" % t.error), file=sys.stderr, flush=True)
-    return r.best
+    return dict(dict(parts=0, no_persist=False, reps_per_block=0, block=0, kernel=0), **r.best)
 
 
 def timing_start(sweep: int, every: int) -> int:
@@ -240,11 +240,11 @@ def sweep_workload(a):
     b = wl.batch
     b.init()
     opts = tuned_options(a, b, wl.pt_every)
-    b.run(0, Wm, 1, pt_every=wl.pt_every, block=a.block, **opts)
+    b.run(0, Wm, 1, pt_every=wl.pt_every, **opts)
     torch.cuda.synchronize()
 
     def plan(nsw, sweep0, frame0, timing=None):
-        return b.plan(0, nsw, 1, L.METROPOLIS, wl.pt_every, L.KERNEL_AUTO, a.block, opts["reps_per_block"], 0,
+        return b.plan(0, nsw, 1, L.METROPOLIS, wl.pt_every, opts["kernel"], opts["block"], opts["reps_per_block"], 0,
                       sweep0, frame0, opts["parts"], opts["no_persist"], timing)
 
     # timed region: K steps, whole job, no per-launch events.  The warmup ran

@@ -6,7 +6,21 @@ replica, and the planner of libmcgpu deals replicas to workgroups by a fixed
 rule (sites per CU, automatic splits of large replicas, persistent launches
 where every workgroup is resident).  autotune() replaces that rule by a
 measurement: it times a few trial sweeps of each candidate plan on the batch
-itself and returns the fastest.
+itself and returns the fastest.  The plan dimensions (Batch.run options):
+
+  parts           workgroups per LDS replica (the tile: 0 the planner's
+                  rule, 1 whole replicas, 2 or 4 parts with halo groups)
+  no_persist      persistent launches (barrier and tempering inside) or not
+  reps_per_block  the planner's packing of replicas per workgroup, or one
+  block           threads per workgroup of the LDS kernel (0: the planner's
+                  1024, or 256 / 512)
+  kernel          the LDS-resident kernel or the streaming colour pass
+
+autotune() searches them one dimension at a time from the planner's plan
+(coordinate descent, each dimension once in the order above), or times an
+explicit candidate list.  The colour schedule is the graph's colouring
+(engine.compile_graph); any proper colouring samples the same distribution,
+but it is part of the trajectory, so it is not a plan option.
 
 The choice does not change the trajectory.  Philox counters are keyed by
 replica, site group, sweep and colour, never by workgroup or thread, so every
@@ -30,10 +44,20 @@ from .engine import Batch
 
 # (parts, no_persist, reps_per_block): automatic splits or none or two parts,
 # persistent launches on or off, the planner's packing or one replica per
-# workgroup
+# workgroup (an explicit grid for autotune(candidates=...))
 DEFAULT_CANDIDATES: Sequence[Dict] = tuple(
     dict(parts=p, no_persist=np_, reps_per_block=rb)
     for p in (0, 1, 2) for np_ in (False, True) for rb in (0, 1))
+
+# the dimensions of the coordinate search, each with its values (the first
+# is the planner's default)
+DIMENSIONS: Sequence = (
+    ("parts", (0, 1, 2, 4)),
+    ("no_persist", (False, True)),
+    ("reps_per_block", (0, 1)),
+    ("block", (0, 256, 512)),
+    ("kernel", (L.KERNEL_AUTO, L.KERNEL_STREAM)),
+)
 
 
 @dataclass
@@ -57,17 +81,17 @@ def _mutable(b: Batch) -> List[torch.Tensor]:
 
 def autotune(b: Batch, sweeps_per_frame: int = 1, pt_every: int = 0, trial_sweeps: int = 64,
              candidates: Optional[Sequence[Dict]] = None) -> TuneResult:
-    """Time each candidate's Batch.run options on trial_sweeps measured sweeps
-    (frames every sweeps_per_frame sweeps, tempering every pt_every sweeps, as
-    the production run will have them) and return the fastest.  Each candidate
-    runs once untimed, then once timed with HIP events on the current stream.
+    """Time Batch.run options on trial_sweeps measured sweeps (frames every
+    sweeps_per_frame sweeps, tempering every pt_every sweeps, as the
+    production run will have them) and return the fastest: the coordinate
+    search over DIMENSIONS, or every plan of `candidates`.  Each plan runs
+    once untimed, then once timed with HIP events on the current stream.
     A candidate the planner rejects, or whose run raises the device error flag,
     is skipped.  The batch's spins, tracked state, accumulators, bins, ladders
     and error flag are as before the call when it returns."""
     if trial_sweeps < 1 or sweeps_per_frame < 1:
         raise ValueError("trial_sweeps and sweeps_per_frame must be >= 1")
-    cands = list(DEFAULT_CANDIDATES if candidates is None else candidates)
-    if not cands:
+    if candidates is not None and not list(candidates):
         raise ValueError("no candidate plans")
     nframe = max(1, trial_sweeps // sweeps_per_frame)
     bufs = _mutable(b)
@@ -77,32 +101,51 @@ def autotune(b: Batch, sweeps_per_frame: int = 1, pt_every: int = 0, trial_sweep
     b.set_nframes(nframe)
     s = torch.cuda.current_stream()
     out = TuneResult(best={})
-    seen = set()
-    try:
-        for opt in cands:
-            key = tuple(sorted(opt.items()))
-            if key in seen:
-                continue
-            seen.add(key)
-            try:
-                times = []
-                for _ in range(2):
-                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                    e0.record(s)
-                    b.run(0, nframe, sweeps_per_frame, pt_every=pt_every, sweep0=sweep0, **opt)
-                    e1.record(s)
-                    e1.synchronize()
-                    times.append(e0.elapsed_time(e1))
-                    flag = b.error_flag()
-                    for x, y in zip(bufs, saved):
-                        x.copy_(y)
-                    if flag:
-                        raise RuntimeError("device error flag 0x%x" % flag)
-                out.trials.append(Trial(dict(opt), times[-1] / (nframe * sweeps_per_frame)))
-            except RuntimeError as e:
+    seen = {}
+
+    def trial(opt):
+        """ms per sweep of plan opt (None: rejected), each plan timed once."""
+        key = tuple(sorted(opt.items()))
+        if key in seen:
+            return seen[key]
+        ms = None
+        try:
+            times = []
+            for _ in range(2):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(s)
+                b.run(0, nframe, sweeps_per_frame, pt_every=pt_every, sweep0=sweep0, **opt)
+                e1.record(s)
+                e1.synchronize()
+                times.append(e0.elapsed_time(e1))
+                flag = b.error_flag()
                 for x, y in zip(bufs, saved):
                     x.copy_(y)
-                out.trials.append(Trial(dict(opt), None, str(e)))
+                if flag:
+                    raise RuntimeError("device error flag 0x%x" % flag)
+            ms = times[-1] / (nframe * sweeps_per_frame)
+            out.trials.append(Trial(dict(opt), ms))
+        except RuntimeError as e:
+            for x, y in zip(bufs, saved):
+                x.copy_(y)
+            out.trials.append(Trial(dict(opt), None, str(e)))
+        seen[key] = ms
+        return ms
+
+    try:
+        if candidates is not None:
+            for opt in candidates:
+                trial(dict(opt))
+        else:
+            # coordinate descent from the planner's plan
+            cur = {k: vals[0] for k, vals in DIMENSIONS}
+            best_ms = trial(cur)
+            for k, vals in DIMENSIONS:
+                for v in vals[1:]:
+                    opt = dict(cur, **{k: v})
+                    ms = trial(opt)
+                    if ms is not None and (best_ms is None or ms < best_ms):
+                        cur, best_ms = opt, ms
     finally:
         for x, y in zip(bufs, saved):
             x.copy_(y)

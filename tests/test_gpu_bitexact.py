@@ -512,3 +512,42 @@ def test_exact_ising_bitexact(gpu, case, kernel):
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), "spins differ with the zero table"
     for k in range(len(betas)):
         assert mcref.cb_energy(1, nbr, J, S, D, h, ref[k])[0] <= b.state_data()[k, L.S_E] + 0.0
+
+
+@pytest.mark.parametrize("block", [256, 512])
+@pytest.mark.parametrize("case", ["ising_sq16", "xy_sq16", "xy_sq24_pool", "heis_honeycomb"])
+def test_block_sizes_bitexact(gpu, case, block):
+    """The LDS kernel with 256- and 512-thread workgroups (an autotuner
+    dimension) follows the checker bit for bit."""
+    nc, mk, betas, h = CASES[case]
+    Lt = mk()
+    print("case", case, "block", block, flush=True)
+    b, g, init = _mk_batch(nc, Lt, betas, h)
+    nsw = 10
+    b.sweeps(0, nsw, kernel=L.KERNEL_LDS, block=block)
+    got = _gpu_spins(b, len(betas))
+    ref = _cpu_spins(nc, init, "f32")
+    nbr, J, S, D = _lattice_tables(Lt, nc)
+    mcref.cb_sweeps(nc, 0 if nc > 1 else 2, nbr, J, S, D, g.colour, betas, [h] * len(betas),
+                    np.arange(len(betas), dtype=np.uint32), SEED, 0, nsw, ref, ising_table=b.ising_table())
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), "spins differ"
+    assert b.error_flag() == 0
+
+
+@pytest.mark.parametrize("block,parts", [(256, 1), (512, 1), (256, 2), (512, 3)])
+def test_block_sizes_persistent_ladders(gpu, block, parts):
+    """Persistent launches (tempering every sweep) with 256- and 512-thread
+    workgroups, split or not: same spins, rungs and swaps as the planner's
+    1024-thread plan."""
+    def run(blk):
+        graphs, reps, betas, spins = _ladder_batch()
+        b = Batch(2, graphs, reps, betas, [0.02] * len(betas), spins, nbins=4, nframes=30)
+        b.init()
+        b.run(5, 30, 1, pt_every=1, kernel=L.KERNEL_LDS, block=blk, parts=parts)
+        return (np.concatenate([b.replica_spins(k).ravel() for k in range(len(reps))]).astype(np.float32),
+                b.rung.cpu().numpy(), b.who.cpu().numpy(), b.pt_acc.cpu().numpy(), b.error_flag())
+    a, c = run(block), run(0)
+    assert a[-1] == 0 and c[-1] == 0
+    assert np.array_equal(a[0].view(np.uint32), c[0].view(np.uint32))
+    for x, y in zip(a[1:4], c[1:4]):
+        assert np.array_equal(x, y)
