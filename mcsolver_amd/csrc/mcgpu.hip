@@ -2534,6 +2534,17 @@ __global__ void k_pt(mcg_batch B, const int32_t* __restrict__ ladder_row0, const
 // global memory with one work item per (frontier site, link)
 // (heisenbergLib.c:310-439, xyLib.c:256-380, isingLib.c:165-236)
 // ---------------------------------------------------------------------------
+// accesses to a workgroup's private scratch in global memory (Wolff stamps
+// and cluster lists): ordered among the workgroup's threads only
+template <typename T>
+__device__ __forceinline__ T wg_load(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <typename T>
+__device__ __forceinline__ void wg_store(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 template <int NC, typename ST>
 __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __restrict__ rep_list, int64_t cl0,
                                                 int ncl) {
@@ -2546,7 +2557,7 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
   const int row = rp.row0 + B.rung[rep];
   const float beta = B.row_beta[row], h = B.row_h[row];
   const uint32_t key0 = B.seed_lo, key1 = B.seed_hi ^ rp.serial;
-  __shared__ int s_cnt, s_lo, s_hi, s_seed;
+  __shared__ int s_nc[3], s_seed;
   __shared__ float s_r[3];
   __shared__ double s_red[16];
   __shared__ int s_acc;
@@ -2586,20 +2597,23 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
       }
       s_r[0] = rx; s_r[1] = ry; s_r[2] = rz;
       s_seed = seed;
-      atomicExch(stamp + seed, cur);
-      mem[0] = seed;
-      s_cnt = 1;
-      s_lo = 0;
-      s_hi = 1;
+      wg_store(stamp + seed, cur);
+      wg_store(mem, seed);
+      s_nc[0] = s_nc[1] = s_nc[2] = 0;
       // cluster accept draw: x.w
       s_acc = (int)(x.w >> 8);
     }
     __syncthreads();
     const float r0 = s_r[0], r1 = s_r[1], r2 = s_r[2];
-    // level-synchronous growth
-    while (true) {
-      const int lo = s_lo, hi = s_hi;
-      if (lo >= hi) break;
+    // Level-synchronous growth, one workgroup barrier per level: the sites
+    // level k adds go to mem[hi + s_nc[k % 3]++], so the frontier of level
+    // k+1 is [hi, hi + s_nc[k % 3]).  During level k the first thread clears
+    // the counter of level k+1, last read (level k-2's) before the barrier
+    // that ended level k-1.  One workgroup owns the cluster, so the stamps
+    // and the list need workgroup scope only: their accesses stay in the
+    // CU's caches.
+    int lo = 0, hi = 1, cnt = 1;
+    for (int k = 0;; ++k) {
       // one work item per (frontier site, link): every item is one short
       // dependent chain (stamp test, bond draw, stamp exchange)
       const uint32_t zl = (uint32_t)G.zmax;
@@ -2607,7 +2621,7 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
       for (uint32_t it = tid; it < nitem; it += blockDim.x) {
         const uint32_t fq = it / zl;
         const int l = (int)(it - fq * zl);
-        const int o = __hip_atomic_load(mem + lo + (int)fq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int o = wg_load(mem + lo + (int)fq);
         float so[3];
         spin_vec<NC>(sp, o, G.cls, G.clsdat, so);
         const float sdo = -(so[0] * r0 + so[1] * r1 + so[2] * r2);
@@ -2615,7 +2629,7 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
           const uint32_t wd = G.link[(size_t)o * G.zmax + l];
           const int t = (int)(wd & 0xFFFFFFu);
           if (t == o) continue;
-          if (__hip_atomic_load(stamp + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == cur) continue;
+          if (wg_load(stamp + t) == cur) continue;
           const uint32_t cw = G.lcid ? (uint32_t)G.lcid[(size_t)o * G.zmax + l] : (wd >> 24);
           const float* J = G.coup + cw * 9;
           float corr;
@@ -2634,23 +2648,26 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
             const u32x4 x = philox4x32_10(
                 u32x4{(uint32_t)(o * G.zmax + l), (uint32_t)cl, (uint32_t)(cl >> 32), TAG_BOND}, key0, key1);
             if (1.0f - exp_neg(corr) > u24(x.x)) {
-              const uint32_t old = atomicExch(stamp + t, cur);
+              const uint32_t old = __hip_atomic_exchange(stamp + t, cur, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
               if (old != cur) {
-                const int pos = atomicAdd(&s_cnt, 1);
-                __hip_atomic_store(mem + pos, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const int pos = hi + atomicAdd(&s_nc[k % 3], 1);
+                wg_store(mem + pos, t);
               }
             }
           }
         }
       }
+      if (tid == 0) s_nc[(k + 1) % 3] = 0;
       __syncthreads();
-      if (tid == 0) {
-        s_lo = hi;
-        s_hi = s_cnt;
+      const int add = s_nc[k % 3];
+      if (add == 0) {
+        cnt = hi;
+        break;
       }
-      __syncthreads();
+      lo = hi;
+      hi += add;
     }
-    const int cnt = s_cnt;
     // cluster acceptance energy (field / anisotropy correction)
     // O(n): one work item per (cluster site, link) plus one per site for the
     // single-ion and field terms
@@ -2660,7 +2677,7 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
     for (uint32_t it = tid; it < nacc; it += blockDim.x) {
       const uint32_t fq = it / zi;
       const int l = (int)(it - fq * zi);
-      const int b = __hip_atomic_load(mem + (int)fq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const int b = wg_load(mem + (int)fq);
       float sb[3];
       spin_vec<NC>(sp, b, G.cls, G.clsdat, sb);
       if constexpr (NC == 1) {
@@ -2683,7 +2700,7 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
                                    r1 * (J[3] * pt[0] + J[4] * pt[1] + J[5] * pt[2]) +
                                    r2 * (J[6] * pt[0] + J[7] * pt[1] + J[8] * pt[2]));
           d += (double)src;
-          if (__hip_atomic_load(stamp + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == cur)
+          if (wg_load(stamp + t) == cur)
             d += (double)(sdt * (pb[0] * (J[0] * r0 + J[1] * r1 + J[2] * r2) +
                                  pb[1] * (J[3] * r0 + J[4] * r1 + J[5] * r2) +
                                  pb[2] * (J[6] * r0 + J[7] * r1 + J[8] * r2)));
@@ -2721,7 +2738,7 @@ __global__ __launch_bounds__(1024) void k_wolff(mcg_batch B, const int32_t* __re
     __syncthreads();
     if (s_acc) {
       for (int f = tid; f < cnt; f += blockDim.x) {
-        const int b = __hip_atomic_load(mem + f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int b = wg_load(mem + f);
         if constexpr (NC == 1) {
           sp.set(b, 0, -sp.get(b, 0));
         } else {
