@@ -110,6 +110,12 @@ typedef struct mcg_graph {
   const int32_t *lcid;     /* [n*zmax] coupling index per link when there are more
                               than 256 distinct tensors (link words then carry
                               coupling 0); NULL otherwise */
+  /* self_pairs != 0: the correlation pairs are (i, i) once for every site (one
+     orbital per cell, mcMain.py's default ki_s = ki_t): the pair sums of a
+     frame are then sum_i s_i = the tracked magnetisation and
+     sum_i s_i.s_i = pair_s2 = sum_i S_i^2, and no spin is read for them */
+  double pair_s2;
+  int32_t self_pairs, pad2_;
 } mcg_graph;
 /* Ising couplings: coup[9c+0] = J*S_target (folded, used by the sweep and the
  * energy), coup[9c+1] = J (raw, used by the block-spin energy). */

@@ -35,7 +35,8 @@ class Graph(C.Structure):
     _fields_ = [(k, C.c_int32) for k in ("n", "zmax", "ncol", "ncoup", "ncls", "nlat", "ncirc", "nr", "nclu",
                                          "ngroup", "gmax", "npat", "ucid", "pad_")] + \
                [(k, P) for k in ("link", "coup", "cls", "clsdat", "col_ptr", "col_site", "pairs", "circ", "rorb",
-                                 "rclu", "rlink", "groups", "rrank", "pos", "pat", "pattab", "lcid")]
+                                 "rclu", "rlink", "groups", "rrank", "pos", "pat", "pattab", "lcid")] + \
+               [("pair_s2", C.c_double), ("self_pairs", C.c_int32), ("pad2_", C.c_int32)]
 
 
 class Replica(C.Structure):
@@ -59,7 +60,7 @@ class Plan(C.Structure):
                [("sweep0", C.c_int64), ("timing", C.POINTER(C.c_double))]
 
 
-assert C.sizeof(Graph) == 56 + 17 * 8
+assert C.sizeof(Graph) == 56 + 17 * 8 + 16
 assert C.sizeof(Replica) == 40
 assert C.sizeof(Plan) == 64
 

@@ -771,6 +771,14 @@ __device__ __forceinline__ void frame_terms(const mcg_graph& G, const mcg_batch&
   if (flags & MCG_ISING_RELE) E -= st[MCG_S_EREF];
   const double e = E * (1.0 / n);
   const double M0 = st[MCG_S_M], M1 = st[MCG_S_M + 1], M2 = st[MCG_S_M + 2];
+  if (pairs && G.self_pairs) {
+    // pairs (i, i) over every site: sum s_i is the tracked magnetisation,
+    // sum s_i.s_i the sum of the squared spin lengths
+    F[0] = F[3] = M0;
+    F[1] = F[4] = M1;
+    F[2] = F[5] = M2;
+    F[6] = G.pair_s2;
+  }
   double Mf;
   const int ax = NC == 3 ? 2 : 0;
   if constexpr (NC == 1) {
@@ -878,6 +886,7 @@ __device__ void frame_sums(const SV& sp, const FrameCtx& C, const mcg_batch& B, 
   if ((flags & MCG_MEAS_PAIRS) && G.nlat > 0) {
     for (int k = 0; k < RK; ++k) v[k] = 0.0;
     slice(G.nlat, part, np, lo, hi);
+    if (G.self_pairs) hi = lo;  // the sums follow from the state (frame_terms)
     for (int p = lo + lt; p < hi; p += TPR) {
       float a[3], b[3];
       spin_vec<NC>(sp, G.pairs[2 * p], G.cls, C.clsdat, a);
@@ -2854,6 +2863,7 @@ struct RunCtx {
   int device = 0;
   int stream_ncol = 0, stream_maxcnt = 0;  // colours and largest colour class of the streamed graphs
   int frame_np_stream = 1, frame_np_all = 1;  // workgroups per replica of the split measurement
+  bool frame_spins = true;  // a measured frame reads spins (launch_frame)
   // what the plan was made for: a run whose plan or batch differs in any of
   // these is refused
   int32_t k_kernel = 0, k_block = 0, k_rpb = 0, k_parts = 0, k_nopersist = 0, k_pt = 0;
@@ -3364,10 +3374,12 @@ int frame_parts(int nreps, int nmax, int ncu) {
 }
 
 template <int NC, typename ST>
+// spins: a measured frame reads spins (a spin-derived observable other than
+// self pairs of an O(n) model); the recompute (mode bit 1) always does
 int launch_frame(const mcg_batch* B, const int32_t* dreps, int nreps, int mode, int64_t frame, double* fsum, int np,
-                 hipStream_t s) {
+                 bool spins, hipStream_t s) {
   if (nreps == 0) return 0;
-  if (mode & 5) {
+  if ((mode & 1) || ((mode & 4) && spins)) {
     const int64_t items = (int64_t)np * nreps;
     if (items > (1 << 30)) return fail("split measurement: %lld work items", (long long)items);
     hipLaunchKernelGGL((k_frame_part<NC, ST>), dim3((unsigned)xcd_grid((int)items)), dim3(256), 0, s, *B, dreps,
@@ -3432,7 +3444,7 @@ int stream_sweeps(const mcg_batch* B, const mcg_plan* P, RunCtx& X, int64_t k0, 
       if (plain) {
         for (int i = 0; i < nrs; ++i)
           if (launch_measure<NC, ST>(B, dreps + i, 1, mode, frame, s)) return -1;
-      } else if (launch_frame<NC, ST>(B, dreps, nrs, mode, frame, X.fsum, X.frame_np_stream, s)) {
+      } else if (launch_frame<NC, ST>(B, dreps, nrs, mode, frame, X.fsum, X.frame_np_stream, X.frame_spins, s)) {
         return -1;
       }
     }
@@ -3495,6 +3507,12 @@ int prepare_typed(const mcg_batch* B, const mcg_plan* P, RunCtx& X) {
     int nmax = 1;
     for (int gi = 0; gi < B->ngraph; ++gi) nmax = std::max(nmax, B->graphs_host[gi].n);
     X.frame_np_stream = frame_parts((int)X.hstream.size(), nmax, X.ncu);
+    // the frame reads spins unless its only spin-derived observable is the
+    // self pairs of an O(n) model
+    X.frame_spins = (B->meas & (MCG_MEAS_Q | MCG_MEAS_RNORM | MCG_MEAS_GROUPS)) != 0;
+    if (B->meas & MCG_MEAS_PAIRS)
+      for (int gi = 0; gi < B->ngraph; ++gi)
+        X.frame_spins = X.frame_spins || B->nc == 1 || !B->graphs_host[gi].self_pairs;
     X.frame_np_all = frame_parts(B->nrep, nmax, X.ncu);
   }
   X.lds_updates_per_sweep = 0;
@@ -3691,10 +3709,10 @@ int exec_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s, RunCtx& X) 
       const int mode = 1 | 8 | (therm_end ? 2 : 0) | (meas ? 4 : 0);
       if (meas || therm_end || k + 1 == total) {
         const int64_t frame = meas ? (k - nth + 1) / spf - 1 + P->frame0 : -1;
-        if (launch_frame<NC, ST>(B, X.all_reps, nall, mode, frame, X.fsum, X.frame_np_all, s)) return -1;
+        if (launch_frame<NC, ST>(B, X.all_reps, nall, mode, frame, X.fsum, X.frame_np_all, X.frame_spins, s)) return -1;
       }
       if (pt_on && ((P->sweep0 + k + 1) % P->pt_every) == 0) {
-        if (launch_frame<NC, ST>(B, X.all_reps, nall, 1 | 8, -1, X.fsum, X.frame_np_all, s)) return -1;
+        if (launch_frame<NC, ST>(B, X.all_reps, nall, 1 | 8, -1, X.fsum, X.frame_np_all, X.frame_spins, s)) return -1;
         hipLaunchKernelGGL(k_pt, dim3((unsigned)X.hl_row0.size()), dim3(64), 0, s, *B, X.lad_row0, X.lad_n,
                            (int)X.hl_row0.size(), P->sweep0 + k);
         HIPCHK(hipGetLastError());
@@ -3752,7 +3770,7 @@ int exec_typed(const mcg_batch* B, const mcg_plan* P, hipStream_t s, RunCtx& X) 
     if (stream_sweeps<NC, ST>(B, P, X, k, ns, nth, spf, plain, s, T)) return -1;
     if (rec_host && recompute_due(*B, P->sweep0 + k + ns - 1)) {
       const int mode = 1 | (k + ns == nth ? 2 : 0);  // recompute (+ energy reference at thermalisation end)
-      if (launch_frame<NC, ST>(B, X.all_reps, nall, mode, -1, X.fsum, X.frame_np_all, s)) return -1;
+      if (launch_frame<NC, ST>(B, X.all_reps, nall, mode, -1, X.fsum, X.frame_np_all, X.frame_spins, s)) return -1;
     }
     k += ns;
     if (!X.persist && pt_on && ((P->sweep0 + k) % P->pt_every) == 0) {

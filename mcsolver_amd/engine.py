@@ -26,7 +26,8 @@ _REF_TO_ROWMAJOR = np.array([0, 3, 4, 6, 1, 5, 7, 8, 2])
 
 
 def colour_graph(nbr: np.ndarray) -> np.ndarray:
-    """Proper colouring of the site graph (native greedy, libmcgpu)."""
+    """Proper colouring of the site graph (libmcgpu: greedy in site order, or
+    DSatur where that needs fewer colours)."""
     nbr = np.ascontiguousarray(nbr, dtype=np.int32)
     n = nbr.shape[0]
     z = nbr.shape[1] if nbr.ndim == 2 else 0
@@ -291,6 +292,13 @@ def upload_graph(g: GraphTables, device) -> None:
     s.gmax = g.gmax
     s.npat = g.npat
     s.ucid = g.ucid
+    # every site paired with itself once: the frame's pair sums follow from
+    # the tracked magnetisation and the spin lengths
+    pr = np.asarray(g.pairs).reshape(-1, 2)
+    if len(pr) == g.n and g.n > 0 and np.array_equal(pr[:, 0], pr[:, 1]) and \
+            np.array_equal(np.sort(pr[:, 0]), np.arange(g.n)):
+        sl = np.asarray(g.clsdat, np.float32).reshape(-1, 8)[:, 0].astype(np.float64)[np.asarray(g.cls)]
+        s.self_pairs, s.pair_s2 = 1, float(np.sum(sl * sl))
     for k, t in d.items():
         setattr(s, k, t.data_ptr())
     g.struct = s

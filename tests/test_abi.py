@@ -32,4 +32,7 @@ def test_struct_sizes_match_header_layout():
     assert C.sizeof(L.Batch) == 48 + 17 * 8 + 8 + 4 + 4
     assert L.Batch.ithr.offset == 48 + 17 * 8
     assert C.sizeof(L.Plan) == 64
+    # mcg_graph: 14 int32 (56 B) + 17 pointers, then pair_s2 and two int32
+    assert C.sizeof(L.Graph) == 56 + 17 * 8 + 8 + 8
+    assert L.Graph.pair_s2.offset == 56 + 17 * 8
     assert C.sizeof(L.Replica) == 40
