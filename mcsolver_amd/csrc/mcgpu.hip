@@ -1466,7 +1466,7 @@ __device__ __forceinline__ double bperm_f64(double x, int src) {
 
 // One exported site of a split replica: its NC components (raw bits, SoA in
 // LDS) as one device-scope store when they fill 4 or 8 bytes, else one per
-// component; import_site is the matching load.
+// component.
 template <int NC, typename U>
 __device__ __forceinline__ void export_site(U* dst, const U* lraw, int n, int site) {
   constexpr int W = NC * (int)sizeof(U);
@@ -1481,17 +1481,69 @@ __device__ __forceinline__ void export_site(U* dst, const U* lraw, int n, int si
     for (int q = 0; q < NC; ++q) dev_store(dst + q, lraw[(size_t)q * n + site]);
   }
 }
+// The other parts' exported sites, into this part's LDS copy (replicas
+// split into three or more parts).  The parts' lists form one round-robin:
+// thread lt takes flat indices lt, lt + TPR, ... over the lists of parts
+// 0..np-1 (its own list has no entries here), so that it imports about
+// (sites of all lists) / TPR sites in all instead of working through every
+// part's list in turn, one device-scope round trip per list.  K sites are
+// loaded before any is stored (K = 1 measured faster than 4 on the
+// honeycomb set).  xs: part 0's export area of this replica and sweep
+// parity; parts are max_export * NC words apart.  dsc: this part's
+// descriptor; part p's is (p - part) * ds ints away.
 template <int NC, typename U>
-__device__ __forceinline__ void import_site(const U* src, U* lraw, int n, int site) {
+__device__ __forceinline__ void import_parts(const U* xs, int max_export, const int32_t* halo, const int32_t* dsc,
+                                             int ds, int ncol, int part, int np, U* lraw, int n, int lt, int TPR) {
   constexpr int W = NC * (int)sizeof(U);
-  if constexpr (W == 8 || W == 4) {
-    typedef typename std::conditional<W == 8, uint64_t, uint32_t>::type P;
-    const P w = dev_load((const P*)src);
+  constexpr bool PACKED = W == 8 || W == 4;
+  typedef typename std::conditional<W == 8, uint64_t, uint32_t>::type P;
+  constexpr int K = 1;
+  int ofs[MAX_PARTS], lst[MAX_PARTS], tot = 0;
 #pragma unroll
-    for (int q = 0; q < NC; ++q) lraw[(size_t)q * n + site] = (U)(w >> (q * 8 * sizeof(U)));
-  } else {
+  for (int p = 0; p < MAX_PARTS; ++p) {
+    const int32_t* d = dsc + (p - part) * ds;
+    const bool other = p < np && p != part;
+    ofs[p] = tot;
+    lst[p] = other ? d[2 * ncol] : 0;
+    tot += other ? d[2 * ncol + 1] : 0;
+  }
+  for (int j0 = lt; j0 < tot; j0 += K * TPR) {
+    P w[K];
+    U v[K][NC];
+    int site[K];
 #pragma unroll
-    for (int q = 0; q < NC; ++q) lraw[(size_t)q * n + site] = dev_load(src + q);
+    for (int u = 0; u < K; ++u) {
+      const int j = j0 + u * TPR;
+      site[u] = -1;
+      if (j < tot) {
+        // the part of flat index j: the last one whose list starts at or
+        // before j (a part with an empty list starts where the next one does)
+        int p = 0;
+#pragma unroll
+        for (int q = 1; q < MAX_PARTS; ++q)
+          if (ofs[q] <= j) p = q;
+        const int i = j - ofs[p];
+        const U* src = xs + ((size_t)p * max_export + i) * NC;
+        site[u] = halo[lst[p] + i];
+        if constexpr (PACKED) {
+          w[u] = dev_load((const P*)src);
+        } else {
+#pragma unroll
+          for (int q = 0; q < NC; ++q) v[u][q] = dev_load(src + q);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+      if (site[u] < 0) continue;
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        if constexpr (PACKED)
+          lraw[(size_t)q * n + site[u]] = (U)(w[u] >> (q * 8 * sizeof(U)));
+        else
+          lraw[(size_t)q * n + site[u]] = v[u][q];
+      }
+    }
   }
 }
 
@@ -1700,12 +1752,18 @@ __device__ __forceinline__ void sweep_colours(const mcg_graph& G, const Spins<ST
     const int q1 = split ? (int)((int64_t)ngrp * (pi.part + 1) / pi.np) : ngrp;
     const int nx = split ? pi.hx[2 * c + 1] : 0;
     const int32_t* xl = split ? pi.halo + pi.hx[2 * c] : nullptr;
+    // A split part's recomputed groups continue the round-robin of its own
+    // groups: a thread whose own loop ends at q (the first q >= q1 of its
+    // deal) takes xl[q - q1], xl[q - q1 + TPR], ...  They fill the part's
+    // last own round: a round of them alone would cost about as much as a
+    // full one (one wave's dependent chain).
     float xE = 0.0f, xM[3] = {0.0f, 0.0f, 0.0f};
     if constexpr (MODE == 0 || MODE == 3) {
       constexpr bool UNI = MODE == 3;
       const float* cp = UNI ? coup + G.ucid * 9 : coup;
       const int qm = q1 - pl.rem;  // the pool takes the last rem groups (unsplit replicas only)
-      for (int q = q0 + lt; q < qm; q += TPR)
+      int q = q0 + lt;
+      for (; q < qm; q += TPR)
         metro_group_pat<NC, ST, UNI>(sp, b, cnt, q, c, s_pid, s_pat, zmax, cp, clsdat, beta, h, kk, key0, key1, fE,
                                      fM, aniso);
       if (pl.pr >= 0) {
@@ -1716,30 +1774,32 @@ __device__ __forceinline__ void sweep_colours(const mcg_graph& G, const Spins<ST
                                      beta.at(pv.x, __float_as_int(pv.w)), pv.y, kk, key0,
                                      __float_as_uint(pv.z), pe, pm, aniso);
         // the first pool group after a tail stores, later ones add
-        double* q = pl.pp + threadIdx.x;
+        double* pp = pl.pp + threadIdx.x;
         const bool first = pfresh && c == 0;
-        q[0] = first ? (double)pe : q[0] + (double)pe;
+        pp[0] = first ? (double)pe : pp[0] + (double)pe;
 #pragma unroll
-        for (int k = 1; k <= NC; ++k) q[k * pl.nr] = first ? (double)pm[k - 1] : q[k * pl.nr] + (double)pm[k - 1];
+        for (int k = 1; k <= NC; ++k) pp[k * pl.nr] = first ? (double)pm[k - 1] : pp[k * pl.nr] + (double)pm[k - 1];
       }
-      for (int j = lt; j < nx; j += TPR)
+      for (int j = q - qm; j < nx; j += TPR)
         metro_group_pat<NC, ST, UNI>(sp, b, cnt, xl[j], c, s_pid, s_pat, zmax, cp, clsdat, beta, h, kk, key0, key1,
                                      xE, xM, aniso);
     } else if constexpr (MODE == 1) {
       const GAS int32_t* lcid = G.lcid ? gas(G.lcid) : nullptr;
       if (G.ucid >= 0) {
         const float* cp = coup + G.ucid * 9;
-        for (int q = q0 + lt; q < q1; q += TPR)
+        int q = q0 + lt;
+        for (; q < q1; q += TPR)
           metro_group<NC, true>(sp, b, cnt, q, c, gas(G.link), lcid, zmax, cp, gas(G.cls), clsdat, beta, h, kk, key0,
                                 key1, fE, fM);
-        for (int j = lt; j < nx; j += TPR)
+        for (int j = q - q1; j < nx; j += TPR)
           metro_group<NC, true>(sp, b, cnt, xl[j], c, gas(G.link), lcid, zmax, cp, gas(G.cls), clsdat, beta, h, kk,
                                 key0, key1, xE, xM);
       } else {
-        for (int q = q0 + lt; q < q1; q += TPR)
+        int q = q0 + lt;
+        for (; q < q1; q += TPR)
           metro_group<NC>(sp, b, cnt, q, c, gas(G.link), lcid, zmax, coup, gas(G.cls), clsdat, beta, h, kk, key0, key1,
                           fE, fM);
-        for (int j = lt; j < nx; j += TPR)
+        for (int j = q - q1; j < nx; j += TPR)
           metro_group<NC>(sp, b, cnt, xl[j], c, gas(G.link), lcid, zmax, coup, gas(G.cls), clsdat, beta, h, kk, key0,
                           key1, xE, xM);
       }
@@ -1748,17 +1808,19 @@ __device__ __forceinline__ void sweep_colours(const mcg_graph& G, const Spins<ST
       const bool uni = G.ucid >= 0;
       const GAS float* cp = gas(coup) + (uni ? G.ucid * 9 : 0);
       if (uni) {
-        for (int q = q0 + lt; q < q1; q += TPR)
+        int q = q0 + lt;
+        for (; q < q1; q += TPR)
           metro_group<NC, true>(sp, b, cnt, q, c, gas(G.link), lcid, zmax, cp, gas(G.cls), gas(clsdat), beta, h, kk,
                                 key0, key1, fE, fM);
-        for (int j = lt; j < nx; j += TPR)
+        for (int j = q - q1; j < nx; j += TPR)
           metro_group<NC, true>(sp, b, cnt, xl[j], c, gas(G.link), lcid, zmax, cp, gas(G.cls), gas(clsdat), beta, h,
                                 kk, key0, key1, xE, xM);
       } else {
-        for (int q = q0 + lt; q < q1; q += TPR)
+        int q = q0 + lt;
+        for (; q < q1; q += TPR)
           metro_group<NC>(sp, b, cnt, q, c, gas(G.link), lcid, zmax, cp, gas(G.cls), gas(clsdat), beta, h, kk, key0,
                           key1, fE, fM);
-        for (int j = lt; j < nx; j += TPR)
+        for (int j = q - q1; j < nx; j += TPR)
           metro_group<NC>(sp, b, cnt, xl[j], c, gas(G.link), lcid, zmax, cp, gas(G.cls), gas(clsdat), beta, h, kk,
                           key0, key1, xE, xM);
       }
@@ -2091,14 +2153,33 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
           pe = dev_load(es + lane);
           for (int q = 0; q < 3; ++q) pm[q] = dev_load(ms + lane * 3 + q);
         }
-        for (int p = 0; p < pi.np; ++p) {
-          if (p == pi.part) continue;
-          const int32_t* hq = pi.hx + (p - pi.part) * DS;
-          const int32_t* ex = a.halo + hq[2 * G.ncol];
-          const int nex = hq[2 * G.ncol + 1];
-          const U* xi = xb + (((size_t)par * a.nslots + w.slot) * MAX_PARTS + p) * (size_t)a.max_export * NC;
-          for (int i = lt; i < nex; i += TPR)
-            import_site<NC>(xi + (size_t)i * NC, lraw, n, ex[i]);
+        if (pi.np == 2) {
+          // the other part's list directly (measured faster than the flat
+          // round-robin on the XY scan's two-part replicas)
+          for (int p = 0; p < pi.np; ++p) {
+            if (p == pi.part) continue;
+            const int32_t* hq = pi.hx + (p - pi.part) * DS;
+            const int32_t* ex = a.halo + hq[2 * G.ncol];
+            const int nex = hq[2 * G.ncol + 1];
+            const U* xi = xb + (((size_t)par * a.nslots + w.slot) * MAX_PARTS + p) * (size_t)a.max_export * NC;
+            for (int i = lt; i < nex; i += TPR) {
+              const U* src = xi + (size_t)i * NC;
+              const int site = ex[i];
+              constexpr int W = NC * (int)sizeof(U);
+              if constexpr (W == 8 || W == 4) {
+                typedef typename std::conditional<W == 8, uint64_t, uint32_t>::type P;
+                const P w = dev_load((const P*)src);
+#pragma unroll
+                for (int q = 0; q < NC; ++q) lraw[(size_t)q * n + site] = (U)(w >> (q * 8 * sizeof(U)));
+              } else {
+#pragma unroll
+                for (int q = 0; q < NC; ++q) lraw[(size_t)q * n + site] = dev_load(src + q);
+              }
+            }
+          }
+        } else {
+          import_parts<NC>(xb + ((size_t)par * a.nslots + w.slot) * MAX_PARTS * (size_t)a.max_export * NC,
+                           a.max_export, a.halo, pi.hx, DS, G.ncol, pi.part, pi.np, lraw, n, lt, TPR);
         }
         if (w0) {
           double E = st[MCG_S_E] + v[0], dm[3] = {v[1], v[2], v[3]};
@@ -3048,7 +3129,7 @@ int halo_tables(const mcg_graph& G, int gd, int np, bool limit, std::vector<int3
 // and by a work target t (site updates per workgroup per sweep): t starts at
 // the batch's updates per sweep over the CU count, so that the slowest
 // workgroup carries about one CU's share.  A replica of n >= 8192 sites
-// larger than t is split over P = min(4, ceil(n/t), n/4096) workgroups
+// larger than t is split over P = min(4, ceil(n/t), n/2048) workgroups
 // (parts); a launch with parts does one sweep.  The plan keeps at most one
 // workgroup per CU (t grows by 5 % until it does), because the LDS the
 // largest item asks for is given to every workgroup.  P->parts = 1 disables
@@ -3111,10 +3192,10 @@ int plan_lds(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool force_stream
   // launch with split replicas runs one sweep and restages them every sweep,
   // which costs more than the imbalance it removes.  With spin-derived
   // observables a persistent launch needs pattern links (modes 0 and 3).
-  // Only two-colour graphs: a part recomputes every group its own sites
-  // depend on through lower colours, a cone that widens by a link per
-  // colour (the 6-colour honeycomb with second neighbours: 34.1 us per
-  // sweep unsplit, 36.4 us in 2 parts, 36.3 us in 4).
+  // A part recomputes every group its own sites depend on through lower
+  // colours, a cone that widens by a link per colour; halo_tables refuses
+  // the split when that is more than a quarter of the part's own groups.
+  // Parts keep at least 2048 sites.
   const bool auto_split = !P->no_persist;
   const bool spin_obs = (B->meas & MEAS_SPIN_OBS) != 0;
   struct Split {
@@ -3139,9 +3220,9 @@ int plan_lds(const mcg_batch* B, const mcg_plan* P, RunCtx& X, bool force_stream
         int np = 1;
         if (P->parts >= 2)
           np = P->parts;
-        else if (P->parts == 0 && auto_split && !nosplit[gi] && G.n >= 8192 && G.n > t && G.ncol <= 2 &&
+        else if (P->parts == 0 && auto_split && !nosplit[gi] && G.n >= 8192 && G.n > t &&
                  (!spin_obs || lds_mode(G) == 0 || lds_mode(G) == 3))
-          np = std::min({MAX_PARTS, (int)std::ceil(G.n / t), G.n / 4096});
+          np = std::min({MAX_PARTS, (int)std::ceil(G.n / t), G.n / 2048});
         if (np >= 2) {
           gnp[gi] = np;
           for (int r : rs) splits.push_back(Split{gi, r, np});
