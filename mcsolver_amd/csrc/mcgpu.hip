@@ -868,16 +868,36 @@ __device__ __forceinline__ void slice(int cnt, int part, int np, int& lo, int& h
   hi = (int)((int64_t)cnt * (part + 1) / np);
 }
 
+// Sites owned by part `part` of a replica split into np parts: in each
+// colour class, the sites of Philox groups [ngrp * part / np,
+// ngrp * (part + 1) / np), as the split sweep deals them.
+struct PartOwn {
+  const int32_t* colptr;  // colour class starts (ncol + 1)
+  int ncol, gd, part, np;
+  __device__ __forceinline__ bool mine(int e) const {
+    int c = 0;
+    while (c + 1 < ncol && e >= colptr[c + 1]) ++c;
+    const int b = colptr[c], ngrp = (colptr[c + 1] - b + gd - 1) / gd;
+    const int q = (e - b) / gd;
+    // the largest p with floor(ngrp * p / np) <= q
+    return (int)(((int64_t)(q + 1) * np + ngrp - 1) / ngrp) - 1 == part;
+  }
+};
+
 // The spin-derived sums of one frame over share `part` of `np` of every
 // loop (pairs, circuits, block spins, groups), left in F (valid in the
 // group's first thread): F[0..8) pairs, F[8] solid angles, F[16..29) block
-// spins, F[32..) groups.  np = 1: the whole frame.
+// spins, F[32..) groups.  np = 1: the whole frame.  With `own`, the share
+// is instead the pairs and circuits whose first site the part owns (the
+// others being current in its copy), and block spins and groups are not
+// measured.
 static constexpr int FSUMS = 88;  // frame sums per replica (F[0, 88))
 template <int NC, typename SV, bool FULL = true>
 __device__ void frame_sums(const SV& sp, const FrameCtx& C, const mcg_batch& B, int64_t frame, uint32_t key0,
-                           uint32_t key1, float h, int part, int np) {
+                           uint32_t key1, float h, int part, int np, const PartOwn* own = nullptr) {
   const mcg_graph& G = *C.G;
   const int lt = C.lt, TPR = C.TPR;
+  const int sp_part = own ? 0 : part, sp_np = own ? 1 : np;  // contiguous slices
   double* F = C.s_frm;  // leader-only use
   double v[RK];
   const int flags = FULL ? B.meas : (B.meas & ~MEAS_SPIN_OBS);
@@ -885,17 +905,19 @@ __device__ void frame_sums(const SV& sp, const FrameCtx& C, const mcg_batch& B, 
   // --- correlation pairs: S_i, S_j, sum s_i.s_j, last-site spin
   if ((flags & MCG_MEAS_PAIRS) && G.nlat > 0) {
     for (int k = 0; k < RK; ++k) v[k] = 0.0;
-    slice(G.nlat, part, np, lo, hi);
+    slice(G.nlat, sp_part, sp_np, lo, hi);
     if (G.self_pairs) hi = lo;  // the sums follow from the state (frame_terms)
     for (int p = lo + lt; p < hi; p += TPR) {
+      const int ia = G.pairs[2 * p];
+      if (own && !own->mine(ia)) continue;
       float a[3], b[3];
-      spin_vec<NC>(sp, G.pairs[2 * p], G.cls, C.clsdat, a);
+      spin_vec<NC>(sp, ia, G.cls, C.clsdat, a);
       spin_vec<NC>(sp, G.pairs[2 * p + 1], G.cls, C.clsdat, b);
       v[0] += a[0]; v[1] += a[1]; v[2] += a[2];
       v[3] += b[0]; v[4] += b[1]; v[5] += b[2];
       v[6] += (double)(a[0] * b[0] + a[1] * b[1] + a[2] * b[2]);
     }
-    if (lt == 0 && part == 0) {
+    if (lt == 0 && (own ? own->mine(G.pos[G.n - 1]) : part == 0)) {
       float t[3];
       spin_vec<NC>(sp, G.pos[G.n - 1], G.cls, C.clsdat, t);
       v[7] = t[0];
@@ -907,13 +929,14 @@ __device__ void frame_sums(const SV& sp, const FrameCtx& C, const mcg_batch& B, 
   // --- topological charge
   if ((flags & MCG_MEAS_Q) && NC == 3 && G.ncirc > 0) {
     for (int k = 0; k < RK; ++k) v[k] = 0.0;
-    slice(G.ncirc, part, np, lo, hi);
+    slice(G.ncirc, sp_part, sp_np, lo, hi);
     // one spin length for every class: no per-site class lookups
     float il = sqrtf(C.clsdat[4]);
     bool one_len = G.ncls <= 64;
     for (int q = 1; q < G.ncls && one_len; ++q) one_len = C.clsdat[q * 8 + 4] == C.clsdat[4];
     for (int p = lo + lt; p < hi; p += TPR) {
       const int ia = G.circ[3 * p], ib = G.circ[3 * p + 1], ic = G.circ[3 * p + 2];
+      if (own && !own->mine(ia)) continue;
       float a[3], b[3], c[3];
       spin_vec<NC>(sp, ia, G.cls, C.clsdat, a);
       spin_vec<NC>(sp, ib, G.cls, C.clsdat, b);
@@ -928,7 +951,7 @@ __device__ void frame_sums(const SV& sp, const FrameCtx& C, const mcg_batch& B, 
     if (lt == 0) F[8] = v[0];
   }
   // --- block-spin observables
-  if ((flags & MCG_MEAS_RNORM) && G.nr > 0) {
+  if ((flags & MCG_MEAS_RNORM) && G.nr > 0 && !own) {
     for (int k = 0; k < RK; ++k) v[k] = 0.0;
     double w[RK];
     for (int k = 0; k < RK; ++k) w[k] = 0.0;
@@ -998,7 +1021,7 @@ __device__ void frame_sums(const SV& sp, const FrameCtx& C, const mcg_batch& B, 
       for (int k = 0; k < 5; ++k) F[24 + k] = w[k];
   }
   // --- orbital groups (heisenbergLib.c:805-830): 2 groups per reduction
-  const int ng = (flags & MCG_MEAS_GROUPS) ? G.ngroup : 0;
+  const int ng = (flags & MCG_MEAS_GROUPS) && !own ? G.ngroup : 0;
   for (int g0 = 0; g0 < ng; g0 += 2) {
     for (int k = 0; k < RK; ++k) v[k] = 0.0;
     slice(G.gmax, part, np, lo, hi);
@@ -1605,8 +1628,14 @@ __device__ bool gather_frame_sums(const SweepArgs& a, const mcg_batch& B, double
   if (!*s_flag) return false;
   if (threadIdx.x < 64)
     for (int q = threadIdx.x; q < FSUMS; q += 64) {
+      // the parts' values loaded together, then added in part order
+      double y[MAX_PARTS];
+#pragma unroll
+      for (int p = 1; p < MAX_PARTS; ++p) y[p] = p < np ? dev_load(fp + (size_t)p * FSUMS + q) : 0.0;
       double x = F[q];
-      for (int p = 1; p < np; ++p) x += dev_load(fp + (size_t)p * FSUMS + q);
+#pragma unroll
+      for (int p = 1; p < MAX_PARTS; ++p)
+        if (p < np) x += y[p];
       F[q] = x;
     }
   __syncthreads();
@@ -2044,6 +2073,11 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
     // workgroups.
     const bool frames_after = a.has_split || a.pt_every == 1;
     const int fflags = FULL ? B.meas : (B.meas & ~MEAS_SPIN_OBS);  // what a frame measures
+    // A split part measures pairs and circuits from its own copy (those whose
+    // first site it owns; the other sites they read are in its export
+    // lists, so current after the import).  Block spins and orbital groups
+    // read every part's published sites instead.
+    const bool own_frame = !(B.meas & (MCG_MEAS_RNORM | MCG_MEAS_GROUPS));
     int rep_l = 0, row0_l = 0, last_row = -1;
     if (l_grp) {
       const ItemRep irl = recs[w.off + lane];
@@ -2113,7 +2147,7 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
           export_site<NC>(xo + (size_t)i * NC, lraw, n, ex[i]);
         if constexpr (FULL) {
           // the frame reads every part's sites: publish this part's own
-          if (meas)
+          if (meas && !own_frame)
             part_publish<ST>(my, (U*)a.fspin + ((size_t)par * a.nslots + w.slot) * NC * a.fspin_n, n, a.fspin_n,
                              NC, s_colptr, G.ncol, GD, pi.part, pi.np, lt, TPR);
         }
@@ -2193,13 +2227,23 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
             if (therm_end) st[MCG_S_EREF] = st[MCG_S_E];
           }
         }
+        // the imported sites are read by other threads: by the frame, and by
+        // the next sweep's first colour when no frame or tempering pass
+        // (which end with a workgroup barrier) comes between
+        __syncthreads();
       }
       if constexpr (FULL) {
         if (meas && pi.np > 1) {
-          // this part's share of the spin-derived sums, over every part's
-          // published sites; part 0 gathers the others'
-          const DevSpins<ST> dsp{(const U*)a.fspin + ((size_t)par * a.nslots + w.slot) * NC * a.fspin_n, a.fspin_n};
-          frame_sums<NC, DevSpins<ST>, true>(dsp, C, B, fcur, key0, key1, hg, pi.part, pi.np);
+          // this part's share of the spin-derived sums: from its own copy, or
+          // over every part's published sites; part 0 gathers the others'
+          if (own_frame) {
+            const PartOwn po{s_colptr, G.ncol, GD, pi.part, pi.np};
+            frame_sums<NC, Spins<ST>, true>(sp, C, B, fcur, key0, key1, hg, pi.part, pi.np, &po);
+          } else {
+            const DevSpins<ST> dsp{(const U*)a.fspin + ((size_t)par * a.nslots + w.slot) * NC * a.fspin_n,
+                                   a.fspin_n};
+            frame_sums<NC, DevSpins<ST>, true>(dsp, C, B, fcur, key0, key1, hg, pi.part, pi.np);
+          }
           if (!gather_frame_sums(a, B, s_frm, w.slot, pi.part, pi.np, fcur, s_flag)) return;
         }
       }
@@ -3039,8 +3083,10 @@ int set_lds_attr(size_t bytes) {
 //   * extra groups: the other parts' groups of each colour that p's own
 //     sites depend on, closed under "neighbour of lower colour";
 //   * export list: p's own sites that another part reads during a sweep
-//     (a recomputed site or a neighbour of one), which the persistent kernel
-//     hands over between sweeps.
+//     (a recomputed site or a neighbour of one) or in a frame (the second
+//     site of a correlation pair, a vertex of a circuit, whose first site
+//     that part owns), which the persistent kernel hands over between
+//     sweeps.
 // The np descriptors are contiguous, DESC(ncol) ints each: per colour
 // (start, count) of the extra groups, then (start, count) of the export
 // list, all indices into `halo`.  Returns the offset of the first
@@ -3107,6 +3153,18 @@ int halo_tables(const mcg_graph& G, int gd, int np, bool limit, std::vector<int3
       extra += (int64_t)(halo.size() - start);
     }
     if (limit && extra * 4 > own_groups) return -1;
+  }
+  // a part measures the correlation pairs and circuits whose first site it
+  // owns from its own copy: the other sites they read are exported too
+  if (G.nlat > 0 || G.ncirc > 0) {
+    std::vector<int32_t> pr((size_t)2 * G.nlat), cr((size_t)3 * G.ncirc);
+    if (G.nlat > 0 && hipMemcpy(pr.data(), G.pairs, pr.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return -2;
+    if (G.ncirc > 0 && hipMemcpy(cr.data(), G.circ, cr.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return -2;
+    for (int k = 0; k < G.nlat; ++k)
+      if (own[pr[2 * k + 1]] != own[pr[2 * k]]) wanted[pr[2 * k + 1]] = 1;
+    for (int k = 0; k < G.ncirc; ++k)
+      for (int v = 1; v < 3; ++v)
+        if (own[cr[3 * k + v]] != own[cr[3 * k]]) wanted[cr[3 * k + v]] = 1;
   }
   int mx = 0;
   for (int part = 0; part < np; ++part) {
