@@ -868,36 +868,28 @@ __device__ __forceinline__ void slice(int cnt, int part, int np, int& lo, int& h
   hi = (int)((int64_t)cnt * (part + 1) / np);
 }
 
-// Sites owned by part `part` of a replica split into np parts: in each
-// colour class, the sites of Philox groups [ngrp * part / np,
-// ngrp * (part + 1) / np), as the split sweep deals them.
-struct PartOwn {
-  const int32_t* colptr;  // colour class starts (ncol + 1)
-  int ncol, gd, part, np;
-  __device__ __forceinline__ bool mine(int e) const {
-    int c = 0;
-    while (c + 1 < ncol && e >= colptr[c + 1]) ++c;
-    const int b = colptr[c], ngrp = (colptr[c + 1] - b + gd - 1) / gd;
-    const int q = (e - b) / gd;
-    // the largest p with floor(ngrp * p / np) <= q
-    return (int)(((int64_t)(q + 1) * np + ngrp - 1) / ngrp) - 1 == part;
-  }
+// The frame's share of one part of a split replica: the correlation pairs
+// and circuits whose first site the part owns (indices into G.pairs and
+// G.circ, from the split tables), and whether it owns the last site.
+struct PartFrame {
+  const int32_t* pairs;
+  const int32_t* circs;
+  int npairs, ncircs;
+  bool last;
 };
 
 // The spin-derived sums of one frame over share `part` of `np` of every
 // loop (pairs, circuits, block spins, groups), left in F (valid in the
 // group's first thread): F[0..8) pairs, F[8] solid angles, F[16..29) block
 // spins, F[32..) groups.  np = 1: the whole frame.  With `own`, the share
-// is instead the pairs and circuits whose first site the part owns (the
-// others being current in its copy), and block spins and groups are not
-// measured.
+// is instead own's pairs and circuits (their other sites being current in
+// the part's copy), and block spins and groups are not measured.
 static constexpr int FSUMS = 88;  // frame sums per replica (F[0, 88))
 template <int NC, typename SV, bool FULL = true>
 __device__ void frame_sums(const SV& sp, const FrameCtx& C, const mcg_batch& B, int64_t frame, uint32_t key0,
-                           uint32_t key1, float h, int part, int np, const PartOwn* own = nullptr) {
+                           uint32_t key1, float h, int part, int np, const PartFrame* own = nullptr) {
   const mcg_graph& G = *C.G;
   const int lt = C.lt, TPR = C.TPR;
-  const int sp_part = own ? 0 : part, sp_np = own ? 1 : np;  // contiguous slices
   double* F = C.s_frm;  // leader-only use
   double v[RK];
   const int flags = FULL ? B.meas : (B.meas & ~MEAS_SPIN_OBS);
@@ -905,19 +897,19 @@ __device__ void frame_sums(const SV& sp, const FrameCtx& C, const mcg_batch& B, 
   // --- correlation pairs: S_i, S_j, sum s_i.s_j, last-site spin
   if ((flags & MCG_MEAS_PAIRS) && G.nlat > 0) {
     for (int k = 0; k < RK; ++k) v[k] = 0.0;
-    slice(G.nlat, sp_part, sp_np, lo, hi);
+    slice(G.nlat, part, np, lo, hi);
+    if (own) lo = 0, hi = own->npairs;
     if (G.self_pairs) hi = lo;  // the sums follow from the state (frame_terms)
-    for (int p = lo + lt; p < hi; p += TPR) {
-      const int ia = G.pairs[2 * p];
-      if (own && !own->mine(ia)) continue;
+    for (int q = lo + lt; q < hi; q += TPR) {
+      const int p = own ? own->pairs[q] : q;
       float a[3], b[3];
-      spin_vec<NC>(sp, ia, G.cls, C.clsdat, a);
+      spin_vec<NC>(sp, G.pairs[2 * p], G.cls, C.clsdat, a);
       spin_vec<NC>(sp, G.pairs[2 * p + 1], G.cls, C.clsdat, b);
       v[0] += a[0]; v[1] += a[1]; v[2] += a[2];
       v[3] += b[0]; v[4] += b[1]; v[5] += b[2];
       v[6] += (double)(a[0] * b[0] + a[1] * b[1] + a[2] * b[2]);
     }
-    if (lt == 0 && (own ? own->mine(G.pos[G.n - 1]) : part == 0)) {
+    if (lt == 0 && (own ? own->last : part == 0)) {
       float t[3];
       spin_vec<NC>(sp, G.pos[G.n - 1], G.cls, C.clsdat, t);
       v[7] = t[0];
@@ -929,14 +921,15 @@ __device__ void frame_sums(const SV& sp, const FrameCtx& C, const mcg_batch& B, 
   // --- topological charge
   if ((flags & MCG_MEAS_Q) && NC == 3 && G.ncirc > 0) {
     for (int k = 0; k < RK; ++k) v[k] = 0.0;
-    slice(G.ncirc, sp_part, sp_np, lo, hi);
+    slice(G.ncirc, part, np, lo, hi);
+    if (own) lo = 0, hi = own->ncircs;
     // one spin length for every class: no per-site class lookups
     float il = sqrtf(C.clsdat[4]);
     bool one_len = G.ncls <= 64;
     for (int q = 1; q < G.ncls && one_len; ++q) one_len = C.clsdat[q * 8 + 4] == C.clsdat[4];
-    for (int p = lo + lt; p < hi; p += TPR) {
+    for (int q = lo + lt; q < hi; q += TPR) {
+      const int p = own ? own->circs[q] : q;
       const int ia = G.circ[3 * p], ib = G.circ[3 * p + 1], ic = G.circ[3 * p + 2];
-      if (own && !own->mine(ia)) continue;
       float a[3], b[3], c[3];
       spin_vec<NC>(sp, ia, G.cls, C.clsdat, a);
       spin_vec<NC>(sp, ib, G.cls, C.clsdat, b);
@@ -1423,8 +1416,10 @@ struct DevSpins {
 };
 
 // ints per part in the split tables: (start, count) of the extra groups per
-// colour, then (start, count) of the export list
-__host__ __device__ inline int halo_desc_ints(int ncol) { return 2 * ncol + 2; }
+// colour, (start, count) of the export list, of the correlation pairs and
+// of the circuits whose first site the part owns, then 1 if the part owns
+// the last site
+__host__ __device__ inline int halo_desc_ints(int ncol) { return 2 * ncol + 7; }
 
 // The leftover groups of a workgroup whose replicas each have one wave
 // (64 threads per replica, R replicas, every colour with the same number of
@@ -2237,8 +2232,9 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
           // this part's share of the spin-derived sums: from its own copy, or
           // over every part's published sites; part 0 gathers the others'
           if (own_frame) {
-            const PartOwn po{s_colptr, G.ncol, GD, pi.part, pi.np};
-            frame_sums<NC, Spins<ST>, true>(sp, C, B, fcur, key0, key1, hg, pi.part, pi.np, &po);
+            const int32_t* d = pi.hx + 2 * G.ncol;
+            const PartFrame pf{a.halo + d[2], a.halo + d[4], d[3], d[5], d[6] != 0};
+            frame_sums<NC, Spins<ST>, true>(sp, C, B, fcur, key0, key1, hg, pi.part, pi.np, &pf);
           } else {
             const DevSpins<ST> dsp{(const U*)a.fspin + ((size_t)par * a.nslots + w.slot) * NC * a.fspin_n,
                                    a.fspin_n};
@@ -3156,16 +3152,16 @@ int halo_tables(const mcg_graph& G, int gd, int np, bool limit, std::vector<int3
   }
   // a part measures the correlation pairs and circuits whose first site it
   // owns from its own copy: the other sites they read are exported too
-  if (G.nlat > 0 || G.ncirc > 0) {
-    std::vector<int32_t> pr((size_t)2 * G.nlat), cr((size_t)3 * G.ncirc);
-    if (G.nlat > 0 && hipMemcpy(pr.data(), G.pairs, pr.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return -2;
-    if (G.ncirc > 0 && hipMemcpy(cr.data(), G.circ, cr.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return -2;
-    for (int k = 0; k < G.nlat; ++k)
-      if (own[pr[2 * k + 1]] != own[pr[2 * k]]) wanted[pr[2 * k + 1]] = 1;
-    for (int k = 0; k < G.ncirc; ++k)
-      for (int v = 1; v < 3; ++v)
-        if (own[cr[3 * k + v]] != own[cr[3 * k]]) wanted[cr[3 * k + v]] = 1;
-  }
+  std::vector<int32_t> pr((size_t)2 * G.nlat), cr((size_t)3 * G.ncirc);
+  int32_t last = 0;  // position of the last site
+  if (G.nlat > 0 && hipMemcpy(pr.data(), G.pairs, pr.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return -2;
+  if (G.ncirc > 0 && hipMemcpy(cr.data(), G.circ, cr.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return -2;
+  if (n > 0 && hipMemcpy(&last, G.pos + (n - 1), 4, hipMemcpyDeviceToHost) != hipSuccess) return -2;
+  for (int k = 0; k < G.nlat; ++k)
+    if (own[pr[2 * k + 1]] != own[pr[2 * k]]) wanted[pr[2 * k + 1]] = 1;
+  for (int k = 0; k < G.ncirc; ++k)
+    for (int v = 1; v < 3; ++v)
+      if (own[cr[3 * k + v]] != own[cr[3 * k]]) wanted[cr[3 * k + v]] = 1;
   int mx = 0;
   for (int part = 0; part < np; ++part) {
     const size_t d = base + (size_t)part * DS;
@@ -3175,6 +3171,18 @@ int halo_tables(const mcg_graph& G, int gd, int np, bool limit, std::vector<int3
     halo[d + 2 * ncol] = (int32_t)start;
     halo[d + 2 * ncol + 1] = (int32_t)(halo.size() - start);
     mx = std::max(mx, (int)(halo.size() - start));
+    // the part's share of the frame
+    const size_t ps = halo.size();
+    for (int k = 0; k < G.nlat; ++k)
+      if (own[pr[2 * k]] == part) halo.push_back(k);
+    halo[d + 2 * ncol + 2] = (int32_t)ps;
+    halo[d + 2 * ncol + 3] = (int32_t)(halo.size() - ps);
+    const size_t cs = halo.size();
+    for (int k = 0; k < G.ncirc; ++k)
+      if (own[cr[3 * k]] == part) halo.push_back(k);
+    halo[d + 2 * ncol + 4] = (int32_t)cs;
+    halo[d + 2 * ncol + 5] = (int32_t)(halo.size() - cs);
+    halo[d + 2 * ncol + 6] = n > 0 && own[last] == part;
   }
   *max_export = std::max(*max_export, mx);
   return (int)base;
