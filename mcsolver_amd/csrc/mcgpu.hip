@@ -884,7 +884,8 @@ struct PartFrame {
 // spins, F[32..) groups.  np = 1: the whole frame.  With `own`, the share
 // is instead own's pairs and circuits (their other sites being current in
 // the part's copy), and block spins and groups are not measured.
-static constexpr int FSUMS = 88;  // frame sums per replica (F[0, 88))
+static constexpr int FSUMS = 88;     // frame sums per replica (F[0, 88))
+static constexpr int FSUMS_PQ = 9;   // of which pairs and circuits (F[0, 9))
 template <int NC, typename SV, bool FULL = true>
 __device__ void frame_sums(const SV& sp, const FrameCtx& C, const mcg_batch& B, int64_t frame, uint32_t key0,
                            uint32_t key1, float h, int part, int np, const PartFrame* own = nullptr) {
@@ -1417,9 +1418,10 @@ struct DevSpins {
 
 // ints per part in the split tables: (start, count) of the extra groups per
 // colour, (start, count) of the export list, of the correlation pairs and
-// of the circuits whose first site the part owns, then 1 if the part owns
-// the last site
-__host__ __device__ inline int halo_desc_ints(int ncol) { return 2 * ncol + 7; }
+// of the circuits whose first site the part owns, 1 if the part owns the
+// last site, then 1 if no pair or circuit of the replica has sites of two
+// parts (the frame then needs only each part's own sites)
+__host__ __device__ inline int halo_desc_ints(int ncol) { return 2 * ncol + 8; }
 
 // The leftover groups of a workgroup whose replicas each have one wave
 // (64 threads per replica, R replicas, every colour with the same number of
@@ -2068,11 +2070,14 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
     // workgroups.
     const bool frames_after = a.has_split || a.pt_every == 1;
     const int fflags = FULL ? B.meas : (B.meas & ~MEAS_SPIN_OBS);  // what a frame measures
-    // A split part measures pairs and circuits from its own copy (those whose
-    // first site it owns; the other sites they read are in its export
-    // lists, so current after the import).  Block spins and orbital groups
-    // read every part's published sites instead.
-    const bool own_frame = !(B.meas & (MCG_MEAS_RNORM | MCG_MEAS_GROUPS));
+    // A split part measures the pairs and circuits whose first site it owns
+    // from its own copy before the grid barrier, when none of them has
+    // sites of two parts and no block spins or orbital groups are measured;
+    // its sums reach part 0 through the barrier (a.fpart, by sweep parity).
+    // Otherwise every part publishes its sites and, after the barrier, sums
+    // its share over the published copy; part 0 waits for the others' sums.
+    const bool own_frame = FULL && pi.np > 1 && !(B.meas & (MCG_MEAS_RNORM | MCG_MEAS_GROUPS)) &&
+                           pi.hx[2 * G.ncol + 7] != 0;
     int rep_l = 0, row0_l = 0, last_row = -1;
     if (l_grp) {
       const ItemRep irl = recs[w.off + lane];
@@ -2141,6 +2146,16 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
         for (int i = lt; i < nex; i += TPR)
           export_site<NC>(xo + (size_t)i * NC, lraw, n, ex[i]);
         if constexpr (FULL) {
+          if (meas && own_frame) {
+            const int32_t* d = pi.hx + 2 * G.ncol;
+            const PartFrame pf{a.halo + d[2], a.halo + d[4], d[3], d[5], d[6] != 0};
+            frame_sums<NC, Spins<ST>, true>(sp, C, B, fcur, key0, key1, hg, pi.part, pi.np, &pf);
+            if (pi.part != 0) {
+              __syncthreads();  // F is written by the first thread
+              double* fp = a.fpart + (((size_t)par * a.nslots + w.slot) * MAX_PARTS + pi.part) * FSUMS;
+              if ((int)threadIdx.x < FSUMS_PQ) dev_store(fp + threadIdx.x, s_frm[threadIdx.x]);
+            }
+          }
           // the frame reads every part's sites: publish this part's own
           if (meas && !own_frame)
             part_publish<ST>(my, (U*)a.fspin + ((size_t)par * a.nslots + w.slot) * NC * a.fspin_n, n, a.fspin_n,
@@ -2232,15 +2247,26 @@ __global__ __launch_bounds__(1024) void k_lds(mcg_batch B, const WorkItem* __res
           // this part's share of the spin-derived sums: from its own copy, or
           // over every part's published sites; part 0 gathers the others'
           if (own_frame) {
-            const int32_t* d = pi.hx + 2 * G.ncol;
-            const PartFrame pf{a.halo + d[2], a.halo + d[4], d[3], d[5], d[6] != 0};
-            frame_sums<NC, Spins<ST>, true>(sp, C, B, fcur, key0, key1, hg, pi.part, pi.np, &pf);
+            // the other parts' sums, stored before the barrier, added in
+            // part order
+            if (pi.part == 0 && (int)threadIdx.x < FSUMS_PQ) {
+              const double* fp = a.fpart + ((size_t)par * a.nslots + w.slot) * MAX_PARTS * FSUMS + threadIdx.x;
+              double y[MAX_PARTS];
+#pragma unroll
+              for (int p = 1; p < MAX_PARTS; ++p) y[p] = p < pi.np ? dev_load(fp + (size_t)p * FSUMS) : 0.0;
+              double x = s_frm[threadIdx.x];
+#pragma unroll
+              for (int p = 1; p < MAX_PARTS; ++p)
+                if (p < pi.np) x += y[p];
+              s_frm[threadIdx.x] = x;
+            }
+            __syncthreads();
           } else {
             const DevSpins<ST> dsp{(const U*)a.fspin + ((size_t)par * a.nslots + w.slot) * NC * a.fspin_n,
                                    a.fspin_n};
             frame_sums<NC, DevSpins<ST>, true>(dsp, C, B, fcur, key0, key1, hg, pi.part, pi.np);
+            if (!gather_frame_sums(a, B, s_frm, w.slot, pi.part, pi.np, fcur, s_flag)) return;
           }
-          if (!gather_frame_sums(a, B, s_frm, w.slot, pi.part, pi.np, fcur, s_flag)) return;
         }
       }
       // the frame of every replica (first wave, lane g), while the other
@@ -3150,18 +3176,16 @@ int halo_tables(const mcg_graph& G, int gd, int np, bool limit, std::vector<int3
     }
     if (limit && extra * 4 > own_groups) return -1;
   }
-  // a part measures the correlation pairs and circuits whose first site it
-  // owns from its own copy: the other sites they read are exported too
+  // the frame's pairs and circuits: local when none has sites of two parts
   std::vector<int32_t> pr((size_t)2 * G.nlat), cr((size_t)3 * G.ncirc);
   int32_t last = 0;  // position of the last site
   if (G.nlat > 0 && hipMemcpy(pr.data(), G.pairs, pr.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return -2;
   if (G.ncirc > 0 && hipMemcpy(cr.data(), G.circ, cr.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return -2;
   if (n > 0 && hipMemcpy(&last, G.pos + (n - 1), 4, hipMemcpyDeviceToHost) != hipSuccess) return -2;
-  for (int k = 0; k < G.nlat; ++k)
-    if (own[pr[2 * k + 1]] != own[pr[2 * k]]) wanted[pr[2 * k + 1]] = 1;
+  bool local = true;
+  for (int k = 0; k < G.nlat; ++k) local = local && own[pr[2 * k + 1]] == own[pr[2 * k]];
   for (int k = 0; k < G.ncirc; ++k)
-    for (int v = 1; v < 3; ++v)
-      if (own[cr[3 * k + v]] != own[cr[3 * k]]) wanted[cr[3 * k + v]] = 1;
+    for (int v = 1; v < 3; ++v) local = local && own[cr[3 * k + v]] == own[cr[3 * k]];
   int mx = 0;
   for (int part = 0; part < np; ++part) {
     const size_t d = base + (size_t)part * DS;
@@ -3183,6 +3207,7 @@ int halo_tables(const mcg_graph& G, int gd, int np, bool limit, std::vector<int3
     halo[d + 2 * ncol + 4] = (int32_t)cs;
     halo[d + 2 * ncol + 5] = (int32_t)(halo.size() - cs);
     halo[d + 2 * ncol + 6] = n > 0 && own[last] == part;
+    halo[d + 2 * ncol + 7] = local;
   }
   *max_export = std::max(*max_export, mx);
   return (int)base;
@@ -3727,7 +3752,7 @@ int prepare_typed(const mcg_batch* B, const mcg_plan* P, RunCtx& X) {
     X.o_mslot = take(&off, (size_t)2 * B->nrep * MAX_PARTS * 3 * 8);
     X.o_xbuf = take(&off, (size_t)2 * std::max(X.nslots, 1) * MAX_PARTS * std::max(X.max_export, 1) * NC * sizeof(ST));
     X.o_mprev3 = take(&off, (size_t)3 * B->nrow * 8);
-    X.o_fpart = take(&off, (size_t)std::max(X.nslots, 1) * MAX_PARTS * FSUMS * 8);
+    X.o_fpart = take(&off, (size_t)2 * std::max(X.nslots, 1) * MAX_PARTS * FSUMS * 8);
   }
   X.reset_bytes = off - X.reset_off;
   if (X.persist && X.nslots > 0 && (B->meas & MEAS_SPIN_OBS)) {

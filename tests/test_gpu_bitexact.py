@@ -424,13 +424,32 @@ def _honeycomb_ladders():
     return graphs, reps, betas, spins
 
 
+def _honeycomb_ladders_local():
+    """Two 6-rung ladders (L = 16 and 24 cells) of a Heisenberg honeycomb
+    with the default correlation pairs (orbital 0 of every cell with
+    itself: half the sites, so the sums are taken from the spins) and no
+    circuits: no pair has sites of two parts."""
+    from cases import _honeycomb
+    graphs, reps, betas, spins = [], [], [], []
+    rng = np.random.default_rng(23)
+    for gi, L_ in enumerate((16, 24)):
+        Lt = _honeycomb(L_, 5 + gi)
+        p = Lt.corr_pairs
+        assert (p[:, 0] == p[:, 1]).all() and len(p) < Lt.n  # summed from the spins
+        graphs.append(compile_graph(3, Lt.nbr, Lt.J, Lt.S, Lt.D, pairs=p))
+        for r in range(6):
+            reps.append(ReplicaSpec(gi, gi * 6, nrung=6, ladder=gi, rung=r))
+            betas.append(1 / (0.6 + 0.1 * r))
+            v = rng.normal(size=(Lt.n, 3))
+            spins.append(1.5 * v / np.linalg.norm(v, axis=1, keepdims=True))
+    return graphs, reps, betas, spins
+
+
 def _honeycomb_ladders_cross_pairs():
     """Two 6-rung ladders (L = 16 and 24 cells) of a Heisenberg honeycomb
     whose correlation pairs join orbital 0 of a cell to orbital 1 of the cell
     three rows on (ki_s = 0, ki_t = 1, overLat (0, 3, 0)), and one triangle
-    per cell: a split part measures the pairs and triangles whose first site
-    it owns, the others' second sites and vertices reaching it through the
-    export lists."""
+    per cell: pairs and triangles with sites of two parts."""
     graphs, reps, betas, spins = [], [], [], []
     rng = np.random.default_rng(22)
     for gi, L_ in enumerate((16, 24)):
@@ -472,24 +491,28 @@ def _run_full(build, nc, parts, no_persist, pt_every, meas, nth=10, nsw=30):
 @pytest.mark.parametrize("model,parts,pt_every", [("xy", 1, 1), ("xy", 2, 1), ("xy", 3, 0), ("xy", 2, 3),
                                                   ("heis", 1, 1), ("heis", 2, 1), ("heis", 4, 0),
                                                   ("xy_pairs", 2, 1), ("heis_pq", 2, 1), ("heis_pq", 3, 0),
-                                                  ("heis_pq", 4, 1)])
+                                                  ("heis_pq", 4, 1), ("heis_local", 2, 1), ("heis_local", 3, 0),
+                                                  ("heis_local", 4, 1)])
 def test_persistent_spin_observables_match_per_launch(gpu, model, parts, pt_every):
     """Spin-derived observables (pair correlations, block spins; for the
     Heisenberg honeycomb also Q and orbital groups) measured inside the
     persistent launch: unsplit replicas from their LDS copies; split ones,
-    with block spins or groups measured, from every part's published sites,
-    each part summing its share, otherwise (pairs and Q only: xy_pairs,
-    heis_pq with pairs between cells) each part over the pairs and
-    triangles whose first site it owns, from its own copy; part 0 adds the
-    parts' sums in part order.  Same spins, swaps and rungs bit for bit as
-    one launch per sweep, accumulators, bins and group sums to FP64
-    summation order."""
+    when no pair or circuit has sites of two parts and no block spins or
+    groups are measured (xy_pairs, heis_local), each part over the pairs
+    and circuits whose first site it owns, from its own copy before the grid
+    barrier, otherwise (heis, heis_pq with pairs and triangles across
+    parts) from every part's published sites, each part summing its share;
+    part 0 adds the parts' sums in part order.  Same spins, swaps and rungs
+    bit for bit as one launch per sweep, accumulators, bins and group sums
+    to FP64 summation order."""
     if model == "xy":
         build, nc, meas = _ladder_batch, 2, L.MEAS_PAIRS | L.MEAS_RNORM
     elif model == "xy_pairs":
         build, nc, meas = _ladder_batch, 2, L.MEAS_PAIRS
     elif model == "heis_pq":
         build, nc, meas = _honeycomb_ladders_cross_pairs, 3, L.MEAS_PAIRS | L.MEAS_Q
+    elif model == "heis_local":
+        build, nc, meas = _honeycomb_ladders_local, 3, L.MEAS_PAIRS
     else:
         build, nc, meas = _honeycomb_ladders, 3, L.MEAS_PAIRS | L.MEAS_RNORM | L.MEAS_Q | L.MEAS_GROUPS
     a = _run_full(build, nc, parts, True, pt_every, meas)
@@ -505,7 +528,7 @@ def test_persistent_spin_observables_match_per_launch(gpu, model, parts, pt_ever
         np.testing.assert_allclose(b[7], a[7], rtol=1e-10, atol=1e-10)
     # the observables were measured: pair correlations and (Heisenberg) Q
     assert np.abs(a[1][:, L.A_SIJ]).max() > 0
-    if model.startswith("heis"):
+    if model in ("heis", "heis_pq"):
         assert np.abs(a[1][:, L.A_Q]).max() > 0
 
 
