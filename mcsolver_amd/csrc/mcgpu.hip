@@ -566,12 +566,22 @@ static constexpr int RK = 8;  // values per reduction call
 static constexpr int FRM_ACC = 96;  // frame scratch: accumulator staging starts here
 
 // FP64 value moved across lanes by DPP (two 32-bit halves); lanes of rows
-// outside ROWMASK get 0
+// outside ROWMASK get 0.  With every row enabled (the quad permutes and row
+// mirrors used here, where each lane reads a lane of its own row) there is
+// no old value to keep, so the destination is not zeroed first: those two
+// moves per step were a third of the instructions of a wave's FP64 sum,
+// which all waves of a workgroup run at once at the end of every sweep.
 template <int CTRL, int ROWMASK>
 __device__ __forceinline__ double dpp_f64(double x) {
   const int lo = __double2loint(x), hi = __double2hiint(x);
-  const int l2 = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROWMASK, 0xF, false);
-  const int h2 = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROWMASK, 0xF, false);
+  int l2, h2;
+  if constexpr (ROWMASK == 0xF) {
+    l2 = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, true);
+    h2 = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, true);
+  } else {
+    l2 = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROWMASK, 0xF, false);
+    h2 = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROWMASK, 0xF, false);
+  }
   return __hiloint2double(h2, l2);
 }
 
