@@ -925,7 +925,9 @@ __device__ void frame_sums(const SV& sp, const FrameCtx& C, const mcg_batch& B, 
       spin_vec<NC>(sp, G.pos[G.n - 1], G.cls, C.clsdat, t);
       v[7] = t[0];
     }
-    group_sum8(v, C.s_red, C.g, TPR);
+    // self pairs: every other lane holds zeros, so the group reduction (a
+    // microsecond of FP64 DPP steps and barriers per frame) is not needed
+    if (!G.self_pairs) group_sum8(v, C.s_red, C.g, TPR);
     if (lt == 0)
       for (int k = 0; k < 8; ++k) F[k] = v[k];
   }
