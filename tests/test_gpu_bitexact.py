@@ -533,7 +533,7 @@ def test_persistent_spin_observables_match_per_launch(gpu, model, parts, pt_ever
 
 
 EXACT_ISING = {
-    # J = 1, h = 0 (the configs[0] workload's couplings) and a dyadic field
+    # J = 1, h = 0 (the Ising 32^2 benchmark's couplings) and a dyadic field
     "sq16_h0": (lambda: lat.square(16, ising=True), [0.2, 0.44, 0.6, 1.0], 0.0),
     "sq16_h025": (lambda: lat.square(16, J=-0.75, ising=True), [0.3, 0.5], 0.25),
     # two orbitals with spin lengths 1 and 1.5, equal couplings

@@ -109,9 +109,9 @@ def _on_rows(label, pts, g, fx, N, names, rel_floor=0.0, q_floor=1e-6):
 
 
 # ---------------------------------------------------------------------------
-# configs[0]: Ising 32x32, 16 temperatures, 2e4 + 5e4 sweeps
+# Ising 32x32, 16 temperatures, 2e4 + 5e4 sweeps
 # ---------------------------------------------------------------------------
-def test_config0_ising_square32(gpu):
+def test_ising_square32_vs_oracle(gpu):
     fx = _fix("ising_square32")
     nth, nsw = fx["nthermal"], fx["nsweep"]
     Lt = lat.square(32, J=-1.0, ising=True)
@@ -144,15 +144,15 @@ def test_config0_ising_square32(gpu):
         for nm, gg, oo in (("U4", g_u4, o_u4), ("chi", g_chi, o_chi)):
             z = welch_z(gg, oo)
             rows.append((nm, z, "ising32 T=%.4f %-3s gpu %.6f  oracle %.6f" % (T, nm, gg.mean(), oo.mean())))
-    _check(rows, "configs[0] Ising 32^2")
+    _check(rows, "Ising 32^2 x 16 T")
 
 
 # ---------------------------------------------------------------------------
-# configs[3]: triangular 128x128 Neel DMI, on the FP32 streaming path and
+# triangular 128x128 Neel DMI, on the FP32 streaming path and
 # the FP16 LDS path
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("store", ["f32", "f16"])
-def test_config3_triangular_dmi128(gpu, store):
+def test_triangular_dmi128_vs_oracle(gpu, store):
     fx = _fix("tri_dmi128")
     nth, nsw = fx["nthermal"], fx["nsweep"]   # the oracle's schedule: same run-length bias of C and U4
     Lt = W.triangular_neel_dmi(128)
@@ -164,15 +164,15 @@ def test_config3_triangular_dmi128(gpu, store):
     g, _ = _per_run(b, len(pts), R)
     rows = _on_rows("tri128 %s" % store, pts, g, fx, Lt.n, ("E", "C", "M", "U4", "Q"),
                     rel_floor=1e-4 if store == "f32" else 4e-3)
-    _check(rows, "configs[3] triangular DMI 128^2 (%s)" % store)
+    _check(rows, "triangular DMI 128^2 (%s)" % store)
 
 
 # ---------------------------------------------------------------------------
-# configs[4]: cubic Heisenberg, Metropolis and Wolff at 16^3 around T_c;
+# cubic Heisenberg, Metropolis and Wolff at 16^3 around T_c;
 # the 64^3 lattice for spin lengths and the FP64 energy check
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("algorithm", ["metropolis", "wolff"])
-def test_config4_cubic16(gpu, algorithm):
+def test_cubic16_vs_oracle(gpu, algorithm):
     fx = _fix("cubic16")
     Lt = lat.cubic(16, J=-1.0)
     pts = list(fx["points"].items())
@@ -188,10 +188,10 @@ def test_config4_cubic16(gpu, algorithm):
     assert b.error_flag() == 0
     g, _ = _per_run(b, len(pts), R)
     rows = _on_rows("cubic16 %s" % algorithm, pts, g, fx, Lt.n, ("E", "C", "M", "U4"))
-    _check(rows, "configs[4] cubic 16^3 %s" % algorithm)
+    _check(rows, "cubic 16^3 %s" % algorithm)
 
 
-def test_config4_cubic64_lengths_and_energy_check(gpu):
+def test_cubic64_lengths_and_energy_check(gpu):
     """64^3 (streaming path, 3 MB per replica): after 200 Metropolis sweeps
     with the FP64 recompute every 25 sweeps, every spin has unit length
     (one Newton step per accepted update) and the tracked energy never

@@ -1,4 +1,4 @@
-"""configs[2], the batched XY ladder scan (square lattices L = 16 ... 128 in
+"""The batched XY ladder scan (square lattices L = 16 ... 128 in
 one launch, 64-rung parallel-tempering ladders over T in [0.7, 1.2], swap
 every sweep), against the FP64 oracle.
 
@@ -73,6 +73,6 @@ def test_xy_scan_batched_matches_oracle(gpu):
                 ov.std(ddof=1) / np.sqrt(len(ov)), z))
             assert abs(z) < 4.5, (key, nm)
     chi2, dof, lim, f2 = chi2_summary(zs)
-    print("configs[2] XY scan: chi2 = %.1f for %d comparisons (99.9%% quantile %.1f); %.0f%% within 2 sigma" %
+    print("XY ladder scan: chi2 = %.1f for %d comparisons (99.9%% quantile %.1f); %.0f%% within 2 sigma" %
           (chi2, dof, lim, 100 * f2))
     assert chi2 < lim
